@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X GR-ray-tracer hot path (one GPU).
+
+Workload (the production FP64 image configuration): Fishbone-Moncrief torus
+around a Kerr hole with a = 0.9 (r_in = 6, r_pressure_max = 12) on a
+256 x 128 x 128 grid, r log-spaced on [1.5, 100]; a 1024 x 1024 image,
+one ray per pixel centre, camera at r = 1000, inclination 60 deg, +-25 M
+field of view; RK45 with rtol = atol = 1e-8; absorption and emission on
+with both coefficients 1; scattering off; superphoton normalisation 1e8.
+
+A step is one full render: all 1024^2 rays traced from the camera until
+every ray has terminated.  Every timed step traces its own batch of
+initial rays that is already resident in HBM.  The headline value is
+accepted RK45 photon-steps per second of wall time over the timed steps.
+
+Prints one JSON line (see the keys below).  --gpus N > 1 exits: this build
+targets one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "gr-ray-tracer_amd"))
+
+METRIC = ("photon RK45 steps/s, FP64 vs mixed-FP32 @≤1e-4 image L1 err; "
+          "adjoint/fwd ratio")
+
+# Algorithmic FP64 operation counts per unit of work (static counts of
+# + - * / sqrt in the reference's expressions, each counted as one flop):
+# an accepted RK45 step (k1 RHS + one 5-stage attempt + the radius test), a
+# rejected attempt (5 RHS + stage/error work), a fluid-sampled step
+# (position -> cell -> tetrad -> radiative transfer).
+F_ACC = 2.0e3
+F_REJ = 1.73e3
+F_FLUID = 1.1e3
+PEAK_FP64_TFLOPS = 78.6     # MI355X vector FP64, published
+
+
+def workload(args):
+    from grt_amd.fm_torus import fishbone_moncrief
+    spin = 0.9
+    fields = fishbone_moncrief(nr=args.nr, ntheta=args.nth, nphi=args.nph, r_min=1.5,
+                               r_max=100.0, mass=1.0, spin=spin, r_in=6.0,
+                               r_pressure_max=12.0)
+    kw = dict(a_bh=spin, m_bh=1.0, r_term_min=1.0 + np.sqrt(1.0 - spin * spin),
+              r_term_max=1500.0, atol=1e-8, rtol=1e-8, integrator=1, mode=0,
+              absorption_enabled=1, emission_enabled=1, absorption_coefficient=1.0,
+              emission_coefficient=1.0, superphoton_count_normalization=1e8,
+              camera_distance=1000.0, camera_theta=np.pi / 3, camera_phi=0.0,
+              camera_width=50.0, camera_height=50.0)
+    n = args.image
+    u = -25.0 + (np.arange(n) + 0.5) * 50.0 / n
+    U, V = np.meshgrid(u, u, indexing="xy")
+    return fields, kw, U.ravel().copy(), V.ravel().copy()
+
+
+def cpu_baseline(fields, kw, su, sv, sample, threads, max_steps):
+    """The oracle (FP64 C/OpenMP restatement, reference O(n) cell search) on a
+    strided subset of the same rays."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import grt_oracle as O
+    idx = np.linspace(0, len(su) - 1, sample).astype(np.int64)
+    po = O.make_params(**kw)
+    ph = O.init_image(po, su[idx], sv[idx], 0.1)
+    G = O.GridHolder(fields, kw["a_bh"], kw["m_bh"])
+    t0 = time.perf_counter()
+    st = O.trace(po, G, ph, max_steps, 1.0, 0, threads)
+    dt = time.perf_counter() - t0
+    return idx, ph, st, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--image", type=int, default=1024)
+    ap.add_argument("--nr", type=int, default=256)
+    ap.add_argument("--nth", type=int, default=128)
+    ap.add_argument("--nph", type=int, default=128)
+    ap.add_argument("--max-steps", type=int, default=10000)
+    ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quiet", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(photons would shard by id with one image all-reduce, see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import grt_amd
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+
+    def log(*a):
+        if not args.quiet:
+            print("[bench]", *a, file=sys.stderr, flush=True)
+
+    t_setup = time.perf_counter()
+    fields, kw, su, sv = workload(args)
+    log(f"grid {fields['r'].shape} built in {time.perf_counter() - t_setup:.1f}s")
+    ctx = grt_amd.Context(grt_amd.default_params(**kw))
+    ctx.set_grid(fields)
+    n = len(su)
+    su_d = torch.from_numpy(su).to(dev)
+    sv_d = torch.from_numpy(sv).to(dev)
+    proto = {name: torch.empty(n, dtype=(torch.int64 if dt == np.uint64 else
+                                         torch.uint8 if dt == np.uint8 else torch.float64),
+                               device=dev) for name, dt in grt_amd.PHOTON_FIELDS}
+    ctx.init_image(proto, 0.1, su_d, sv_d)
+    torch.cuda.synchronize()
+    batches = [{k: v.clone() for k, v in proto.items()} for _ in range(args.warmup + args.steps)]
+    torch.cuda.synchronize()
+    log(f"{len(batches)} batches of {n} rays resident; setup {time.perf_counter() - t_setup:.1f}s")
+
+    for w in range(args.warmup):
+        st = ctx.trace(batches[w], args.max_steps)
+        log(f"warmup {w}: {st['accepted_steps']} steps, {st['kernel_ms']:.2f} ms kernel")
+
+    totals = dict(accepted_steps=0, rejected_attempts=0, functor_calls=0, fluid_samples=0,
+                  failed_steps=0, global_steps=0, launches=0, kernel_ms=0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        st = ctx.trace(batches[args.warmup + k], args.max_steps)
+        for key in totals:
+            totals[key] += st[key]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    log(f"timed {args.steps} renders in {wall * 1e3:.1f} ms; totals {totals}")
+
+    last = grt_amd.to_host(batches[-1])
+    img = ctx.image_products(batches[-1], args.image, args.image)
+    arrived = img["arrived_count"]
+
+    steps_per_s = totals["accepted_steps"] / wall
+    flops = (totals["accepted_steps"] * F_ACC + totals["rejected_attempts"] * F_REJ +
+             totals["fluid_samples"] * F_FLUID)
+    kern_s = totals["kernel_ms"] * 1e-3
+    avg_launch_ms = totals["kernel_ms"] / max(1, totals["launches"])
+    achieved = flops / kern_s / 1e12 if kern_s > 0 else 0.0
+
+    result = {
+        "metric": METRIC,
+        "value": steps_per_s,
+        "unit": "photon RK45 steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "FP64 image render, FM torus a=0.9 on 256x128x128, "
+                        f"{args.image}x{args.image} rays, inclination 60 deg, RK45 rtol 1e-8, "
+                        "absorption+emission",
+            "grid": [args.nr, args.nth, args.nph],
+            "image": [args.image, args.image],
+            "rays_per_step": n,
+            "precision": "fp64",
+        },
+        "pixels_per_s": n * args.steps / wall,
+        "requested_steps_per_s": n * totals["global_steps"] / wall,
+        "per_render": {k: v / args.steps for k, v in totals.items()},
+        "arrived_count": arrived,
+        "roofline": {
+            "bound": "fp64-valu",
+            "achieved": achieved,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS,
+            "traffic": None,
+            "kernel": "trace_fp64<RK45>",
+            "avg_launch_ms": avg_launch_ms,
+            "flops_per_launch": flops / max(1, totals["launches"]),
+        },
+    }
+
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        idx, ph_o, st_o, dt = cpu_baseline(fields, kw, su, sv, args.cpu_sample, threads,
+                                           args.max_steps)
+        result["cpu_baseline"] = {
+            "value": st_o["accepted_steps"] / dt,
+            "unit": "photon RK45 steps/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"{args.cpu_sample} of the {n} rays (evenly strided), same grid and "
+                      f"settings, {dt:.1f} s wall on {threads} threads",
+        }
+        # parity of the sampled rays against the GPU render of the same rays
+        X = np.stack([last[k][idx] for k in ("x0", "x1", "x2", "x3")])
+        Xo = np.stack([ph_o[k] for k in ("x0", "x1", "x2", "x3")])
+        ex = (np.abs(X - Xo) / np.maximum(np.abs(Xo).max(axis=0), 1e-300)).max(axis=0)
+        result["parity_sample"] = {
+            "rays": int(len(idx)),
+            "phase_equal": float(np.mean(last["phase"][idx] == ph_o["phase"])),
+            "x_rel_err_max": float(ex.max()),
+            "x_within_1e-9": float(np.mean(ex <= 1e-9)),
+        }
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

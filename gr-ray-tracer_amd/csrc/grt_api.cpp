@@ -1,0 +1,698 @@
+// grt_api.cpp -- host side of libgrt_amd.so: the C ABI of include/grt.h.
+//
+// Owns the device copy of the fluid grid (cell records + search tables),
+// the work lists of the persistent tracing kernel and the per-photon RNG
+// counters, and reproduces integrate_geodesics' global-step loop
+// (integrate_cart_ks.hpp:423-535) as a sequence of step-budgeted launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "grt.h"
+#include "grt_host_prep.hpp"
+#include "grt_internal.hpp"
+
+using grt::GridView;
+using grt::PhotonPtrs;
+using grt::StepParams;
+
+struct grt_ctx {
+    grt_params params{};
+    StepParams sp{};
+    grt_tuning tune{256, 2, 256};
+    int device = 0;
+    int num_cus = 0;
+    std::string err;
+
+    // grid
+    bool has_grid = false;
+    int nr = 0, nth = 0, nph = 0;
+    bool separable = false;
+    double* d_tables = nullptr;   // r_axis, r_thr, th_axis, th_thr, ph_axis, ph_thr
+    double* d_coords = nullptr;   // r3, th3, ph3 (general path only)
+    double* d_cells = nullptr;    // kCellDoubles per cell
+    size_t ncell = 0;
+
+    // scratch of the tracing driver
+    uint32_t* d_lists = nullptr;  // two lists of list_cap entries
+    size_t list_cap = 0;
+    uint32_t* d_rng = nullptr;
+    size_t rng_cap = 0;
+    uint32_t* d_counters = nullptr;  // [0] queue, [1] n_out, [2] live count
+    unsigned long long* d_stats = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    // device staging for the host-pointer entry points
+    void* d_stage = nullptr;
+    size_t stage_cap = 0;
+    double* d_img = nullptr;
+    size_t img_cap = 0;
+};
+
+namespace {
+
+std::string g_create_error;
+
+int fail(grt_ctx* ctx, int code, const std::string& msg)
+{
+    if (ctx) ctx->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+int hip_fail(grt_ctx* ctx, hipError_t e, const char* what)
+{
+    return fail(ctx, GRT_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define GRT_HIP(ctx, call)                                  \
+    do {                                                    \
+        const hipError_t e_ = (call);                       \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
+    } while (0)
+
+bool finite(double v) { return std::isfinite(v); }
+
+// Checks of toml_config.hpp:166-277 that concern the traced path.
+int validate_params(grt_ctx* ctx, const grt_params* p)
+{
+    if (!p) return fail(ctx, GRT_E_INVALID, "params must not be NULL");
+    if (!(p->m_bh >= 0.0) || !finite(p->m_bh))
+        return fail(ctx, GRT_E_INVALID, "black_hole.mass must be finite and non-negative");
+    if (!finite(p->a_bh) || std::fabs(p->a_bh) > p->m_bh)
+        return fail(ctx, GRT_E_INVALID,
+                    "black_hole.spin must satisfy |spin| <= mass (a naked singularity otherwise)");
+    if (p->integrator != GRT_RK4 && p->integrator != GRT_RK45)
+        return fail(ctx, GRT_E_INVALID, "integration.integrator must be \"rk4\" or \"rk45\"");
+    if (p->mode != GRT_MODE_IMAGE && p->mode != GRT_MODE_DISK)
+        return fail(ctx, GRT_E_INVALID, "simulation.mode must be 'image' or 'disk'");
+    if (p->precision != GRT_PRECISION_FP64 && p->precision != GRT_PRECISION_MIXED)
+        return fail(ctx, GRT_E_INVALID, "precision must be GRT_PRECISION_FP64 or GRT_PRECISION_MIXED");
+    if (!finite(p->atol) || !finite(p->rtol) || !finite(p->min_step_scale) ||
+        !finite(p->max_step_scale) || !finite(p->safety))
+        return fail(ctx, GRT_E_INVALID, "integration tolerances and step scales must be finite");
+    if (!(p->camera_distance > 0.0) || !(p->camera_width > 0.0) || !(p->camera_height > 0.0))
+        return fail(ctx, GRT_E_INVALID, "camera distance, width, and height must be positive");
+    if (!finite(p->albedo) || p->albedo < 0.0 || p->albedo > 1.0)
+        return fail(ctx, GRT_E_INVALID, "scattering.albedo must be finite and in [0, 1]");
+    if (!finite(p->absorption_coefficient) || p->absorption_coefficient < 0.0)
+        return fail(ctx, GRT_E_INVALID,
+                    "radiative_transfer.absorption_coefficient must be finite and non-negative");
+    if (!finite(p->emission_coefficient) || p->emission_coefficient < 0.0)
+        return fail(ctx, GRT_E_INVALID,
+                    "radiative_transfer.emission_coefficient must be finite and non-negative");
+    // PhotonGenerationConfig::validate, photon_generation.hpp:52-74
+    if (!(p->superphoton_count_normalization > 0.0) || !finite(p->superphoton_count_normalization))
+        return fail(ctx, GRT_E_INVALID,
+                    "photons.superphoton_count_normalization must be finite and positive");
+    if (!(p->energy_per_cell_erg >= 0.0) || !finite(p->energy_per_cell_erg))
+        return fail(ctx, GRT_E_INVALID, "photons.energy_per_cell_erg must be finite and non-negative");
+    if (p->generator == GRT_GEN_POWER_LAW) {
+        if (!finite(p->power_law_slope))
+            return fail(ctx, GRT_E_INVALID, "photons.power_law_slope must be finite");
+        if (!(p->nu_min_hz > 0.0) || !(p->nu_max_hz > p->nu_min_hz) || !finite(p->nu_min_hz) ||
+            !finite(p->nu_max_hz))
+            return fail(ctx, GRT_E_INVALID,
+                        "power-law generation requires 0 < photons.nu_min_hz < photons.nu_max_hz");
+    }
+    if (p->generator == GRT_GEN_CUSTOM &&
+        (!(p->custom_frequency_hz > 0.0) || !finite(p->custom_frequency_hz)))
+        return fail(ctx, GRT_E_INVALID, "photons.custom_frequency_hz must be finite and positive");
+    // UnitConversions::validate, photon_generation.hpp:26-37
+    const double u[6] = {p->length_cm_per_code, p->time_s_per_code, p->density_g_cm3_per_code,
+                         p->temperature_k_per_code, p->four_velocity_cm_s_per_code,
+                         p->magnetic_gauss_per_code};
+    for (double v : u)
+        if (!(v > 0.0) || !finite(v))
+            return fail(ctx, GRT_E_INVALID, "all [units] conversion factors must be finite and positive");
+    return GRT_OK;
+}
+
+void free_grid(grt_ctx* c)
+{
+    if (c->d_tables) (void)hipFree(c->d_tables);
+    if (c->d_coords) (void)hipFree(c->d_coords);
+    if (c->d_cells) (void)hipFree(c->d_cells);
+    c->d_tables = c->d_coords = c->d_cells = nullptr;
+    c->has_grid = false;
+}
+
+int ensure_scratch(grt_ctx* c, uint64_t n)
+{
+    if (n > c->list_cap) {
+        if (c->d_lists) (void)hipFree(c->d_lists);
+        c->d_lists = nullptr;
+        c->list_cap = 0;
+        const size_t cap = std::max<uint64_t>(n, 1024);
+        GRT_HIP(c, hipMalloc(&c->d_lists, 2 * cap * sizeof(uint32_t)));
+        c->list_cap = cap;
+    }
+    if (n > c->rng_cap) {
+        if (c->d_rng) (void)hipFree(c->d_rng);
+        c->d_rng = nullptr;
+        c->rng_cap = 0;
+        const size_t cap = std::max<uint64_t>(n, 1024);
+        GRT_HIP(c, hipMalloc(&c->d_rng, cap * sizeof(uint32_t)));
+        c->rng_cap = cap;
+    }
+    return GRT_OK;
+}
+
+PhotonPtrs to_ptrs(const grt_photons& p, uint32_t* rng)
+{
+    PhotonPtrs d{};
+    d.id = p.id;
+    d.freq = p.frequency;
+    d.eem = p.emission_frame_energy;
+    d.x[0] = p.x0; d.x[1] = p.x1; d.x[2] = p.x2; d.x[3] = p.x3;
+    d.k[0] = p.k0; d.k[1] = p.k1; d.k[2] = p.k2; d.k[3] = p.k3;
+    d.st[0] = p.I; d.st[1] = p.Q; d.st[2] = p.U; d.st[3] = p.V;
+    d.dl = p.dlambda;
+    d.term = p.terminate;
+    d.phase = p.phase;
+    d.udisp = p.theta_disp;
+    d.vdisp = p.phi_disp;
+    d.rng_draw = rng;
+    return d;
+}
+
+bool photons_complete(const grt_photons* p)
+{
+    const void* f[] = {p->id, p->frequency, p->emission_frame_energy, p->x0, p->x1, p->x2, p->x3,
+                       p->k0, p->k1, p->k2, p->k3, p->I, p->Q, p->U, p->V, p->dlambda,
+                       p->terminate, p->phase, p->theta_disp, p->phi_disp};
+    for (const void* q : f)
+        if (!q) return false;
+    return true;
+}
+
+// Carve a device staging block into the 20 photon arrays.
+int stage_photons(grt_ctx* c, uint64_t n, grt_photons* d)
+{
+    const size_t need = n * (18 * sizeof(double) + 2) + 64 * 20;
+    if (need > c->stage_cap) {
+        if (c->d_stage) (void)hipFree(c->d_stage);
+        c->d_stage = nullptr;
+        c->stage_cap = 0;
+        GRT_HIP(c, hipMalloc(&c->d_stage, need));
+        c->stage_cap = need;
+    }
+    char* base = static_cast<char*>(c->d_stage);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char* p = base + off;
+        off += (bytes + 63) & ~size_t(63);
+        return p;
+    };
+    d->id = reinterpret_cast<uint64_t*>(take(n * 8));
+    double** dbl[] = {&d->frequency, &d->emission_frame_energy, &d->x0, &d->x1, &d->x2, &d->x3,
+                      &d->k0, &d->k1, &d->k2, &d->k3, &d->I, &d->Q, &d->U, &d->V, &d->dlambda,
+                      &d->theta_disp, &d->phi_disp};
+    for (double** q : dbl) *q = reinterpret_cast<double*>(take(n * 8));
+    d->terminate = reinterpret_cast<uint8_t*>(take(n));
+    d->phase = reinterpret_cast<uint8_t*>(take(n));
+    return GRT_OK;
+}
+
+int copy_photons(grt_ctx* c, const grt_photons* dst, const grt_photons* src, uint64_t n,
+                 hipMemcpyKind kind)
+{
+    GRT_HIP(c, hipMemcpy(dst->id, src->id, n * 8, kind));
+    const std::pair<double*, double*> dd[] = {
+        {dst->frequency, src->frequency}, {dst->emission_frame_energy, src->emission_frame_energy},
+        {dst->x0, src->x0}, {dst->x1, src->x1}, {dst->x2, src->x2}, {dst->x3, src->x3},
+        {dst->k0, src->k0}, {dst->k1, src->k1}, {dst->k2, src->k2}, {dst->k3, src->k3},
+        {dst->I, src->I}, {dst->Q, src->Q}, {dst->U, src->U}, {dst->V, src->V},
+        {dst->dlambda, src->dlambda}, {dst->theta_disp, src->theta_disp},
+        {dst->phi_disp, src->phi_disp}};
+    for (const auto& q : dd) GRT_HIP(c, hipMemcpy(q.first, q.second, n * 8, kind));
+    GRT_HIP(c, hipMemcpy(dst->terminate, src->terminate, n, kind));
+    GRT_HIP(c, hipMemcpy(dst->phase, src->phase, n, kind));
+    return GRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void grt_params_default(grt_params* p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    p->m_bh = 1.0;                       // utils.cpp:10
+    p->a_bh = 1.0;                       // utils.cpp:11
+    p->r_term_min = 1.0;                 // R_HORIZON for a = M = 1 (utils.cpp:12,27)
+    p->camera_distance = 10000.0;        // utils.cpp:21
+    p->r_term_max = 1.5 * 10000.0;       // utils.cpp:28
+    p->atol = 1e-8;                      // utils.cpp:30
+    p->rtol = 1e-8;                      // utils.cpp:31
+    p->min_step_scale = 0.01;            // utils.hpp:179
+    p->max_step_scale = 10.0;            // utils.hpp:180
+    p->safety = 0.9;                     // utils.hpp:178
+    p->integrator = GRT_RK45;            // simulation_options.hpp:26
+    p->mode = GRT_MODE_IMAGE;            // simulation_options.hpp:14
+    p->albedo = 1.0;                     // simulation_options.hpp:20
+    p->seed = 12345u;                    // simulation_options.hpp:21
+    p->generator = GRT_GEN_BLACKBODY;    // photon_generation.hpp:41
+    p->superphoton_count_normalization = 1.0;
+    p->energy_per_cell_erg = 1.0;
+    p->power_law_slope = 2.0;
+    p->nu_min_hz = 1.0e9;
+    p->nu_max_hz = 1.0e20;
+    p->custom_frequency_hz = 2.30e11;
+    const double pi = 3.14159265358979323846;
+    p->camera_theta = pi / 4.0;          // utils.cpp:15
+    p->camera_phi = pi / 4.0;            // utils.cpp:16
+    p->camera_width = 20.0;              // utils.cpp:17
+    p->camera_height = 20.0;             // utils.cpp:18
+    p->length_cm_per_code = p->time_s_per_code = p->density_g_cm3_per_code = 1.0;
+    p->temperature_k_per_code = p->four_velocity_cm_s_per_code = p->magnetic_gauss_per_code = 1.0;
+    p->precision = GRT_PRECISION_FP64;
+}
+
+int grt_abi_version(void) { return GRT_ABI_VERSION; }
+
+const char* grt_last_error(const grt_ctx* ctx)
+{
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int grt_create(const grt_params* p, grt_ctx** out)
+{
+    if (!out) return fail(nullptr, GRT_E_INVALID, "out must not be NULL");
+    *out = nullptr;
+    const int v = validate_params(nullptr, p);
+    if (v != GRT_OK) return v;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(nullptr, GRT_E_DEVICE, std::string("no HIP device: ") + hipGetErrorString(e));
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) return fail(nullptr, GRT_E_DEVICE, hipGetErrorString(e));
+    grt_ctx* c = new grt_ctx();
+    c->device = dev;
+    c->num_cus = prop.multiProcessorCount;
+    c->params = *p;
+    c->sp = grt::make_step_params(*p);
+    if (hipMalloc(&c->d_counters, 8 * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&c->d_stats, grt::kStatCount * sizeof(unsigned long long)) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        grt_destroy(c);
+        return fail(nullptr, GRT_E_NOMEM, "device allocation failed in grt_create");
+    }
+    *out = c;
+    return GRT_OK;
+}
+
+int grt_destroy(grt_ctx* c)
+{
+    if (!c) return GRT_OK;
+    free_grid(c);
+    void* bufs[] = {c->d_lists, c->d_rng, c->d_counters, c->d_stats, c->d_stage, c->d_img};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    delete c;
+    return GRT_OK;
+}
+
+int grt_set_params(grt_ctx* c, const grt_params* p)
+{
+    if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
+    const int v = validate_params(c, p);
+    if (v != GRT_OK) return v;
+    c->params = *p;
+    c->sp = grt::make_step_params(*p);
+    return GRT_OK;
+}
+
+int grt_get_tuning(grt_ctx* c, grt_tuning* t)
+{
+    if (!c || !t) return fail(c, GRT_E_INVALID, "ctx and tuning must not be NULL");
+    *t = c->tune;
+    return GRT_OK;
+}
+
+int grt_set_tuning(grt_ctx* c, const grt_tuning* t)
+{
+    if (!c || !t) return fail(c, GRT_E_INVALID, "ctx and tuning must not be NULL");
+    if (t->block < 64 || t->block > 256 || t->block % 64 != 0)
+        return fail(c, GRT_E_INVALID, "tuning.block must be 64, 128, 192 or 256");
+    if (t->blocks_per_cu < 1 || t->blocks_per_cu > 32)
+        return fail(c, GRT_E_INVALID, "tuning.blocks_per_cu must be in [1, 32]");
+    if (t->sub_budget < 1)
+        return fail(c, GRT_E_INVALID, "tuning.sub_budget must be positive");
+    c->tune = *t;
+    return GRT_OK;
+}
+
+int grt_set_grid(grt_ctx* c, int32_t nr, int32_t nth, int32_t nph, const double* r,
+                 const double* theta, const double* phi, const double* rho, const double* tgas,
+                 const double* vel, const double* mag)
+{
+    if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
+    if (nr <= 0 || nth <= 0 || nph <= 0)
+        return fail(c, GRT_E_INVALID, "grid extents must be positive");
+    if (!r || !theta || !phi || !rho || !tgas || !vel || !mag)
+        return fail(c, GRT_E_INVALID, "every grid field pointer must be non-NULL");
+    const size_t G = (size_t)nr * nth * nph;
+    if (G > (size_t)1 << 31) return fail(c, GRT_E_INVALID, "grid has more than 2^31 cells");
+    free_grid(c);
+    grt::HostGrid hg;
+    grt::prepare_grid(c->params, nr, nth, nph, r, theta, phi, rho, tgas, vel, mag, hg);
+    c->nr = nr;
+    c->nth = nth;
+    c->nph = nph;
+    c->ncell = G;
+    c->separable = hg.separable;
+    hipError_t e;
+    if (hg.separable) {
+        e = hipMalloc(&c->d_tables, hg.tables.size() * sizeof(double));
+        if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(tables)");
+        e = hipMemcpy(c->d_tables, hg.tables.data(), hg.tables.size() * sizeof(double),
+                      hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(tables)");
+    } else {
+        e = hipMalloc(&c->d_coords, 3 * G * sizeof(double));
+        if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(coords)");
+        const double* src[3] = {r, theta, phi};
+        for (int q = 0; q < 3; ++q) {
+            e = hipMemcpy(c->d_coords + q * G, src[q], G * sizeof(double), hipMemcpyHostToDevice);
+            if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(coords)");
+        }
+    }
+    e = hipMalloc(&c->d_cells, hg.cells.size() * sizeof(double));
+    if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(cells)");
+    e = hipMemcpy(c->d_cells, hg.cells.data(), hg.cells.size() * sizeof(double),
+                  hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(cells)");
+    c->has_grid = true;
+    return GRT_OK;
+}
+
+namespace {
+
+GridView make_grid_view(const grt_ctx* c)
+{
+    return grt::grid_view(c->nr, c->nth, c->nph, c->has_grid, c->separable, c->d_tables,
+                          c->d_coords, c->d_cells);
+}
+
+}  // namespace
+
+int grt_trace_device(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps,
+                     double termination_percent, int64_t output_interval, void* stream_v,
+                     grt_stats* stats)
+{
+    if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
+    if (!dev || !photons_complete(dev)) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
+    if (n > 0xFFFFFFF0ull) return fail(c, GRT_E_INVALID, "more than 2^32 - 16 photons in one call");
+    if (max_steps <= 0) return fail(c, GRT_E_INVALID, "integration.max_steps must be positive");
+    if (!(termination_percent > 0.0) || termination_percent > 1.0)
+        return fail(c, GRT_E_INVALID, "integration.termination_percent must be in (0, 1]");
+    if (c->params.precision != GRT_PRECISION_FP64)
+        return fail(c, GRT_E_INVALID, "the mixed-precision path is not available in this build");
+    hipStream_t stream = static_cast<hipStream_t>(stream_v);
+    GRT_HIP(c, hipSetDevice(c->device));
+    int rc = ensure_scratch(c, n);
+    if (rc != GRT_OK) return rc;
+
+    // Fluid-dependent features are active only with a grid, integrate_cart_ks.hpp:436-452.
+    StepParams P = c->sp;
+    const bool fluid = c->has_grid;
+    P.absorb = P.absorb && fluid;
+    P.emit = P.emit && fluid;
+    P.scatter = P.scatter && fluid;
+
+    grt_stats st{};
+    GRT_HIP(c, hipMemsetAsync(c->d_rng, 0, n * sizeof(uint32_t), stream));
+    GRT_HIP(c, hipMemsetAsync(c->d_stats, 0, grt::kStatCount * sizeof(unsigned long long), stream));
+    GRT_HIP(c, hipMemsetAsync(c->d_counters, 0, 8 * sizeof(uint32_t), stream));
+    uint32_t* list[2] = {c->d_lists, c->d_lists + c->list_cap};
+    GRT_HIP(c, grt::launch_build_live(dev->terminate, n, list[0], c->d_counters + 2, stream));
+    uint32_t live = 0;
+    GRT_HIP(c, hipMemcpyAsync(&live, c->d_counters + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    GRT_HIP(c, hipStreamSynchronize(stream));
+
+    grt::TraceLaunch L{};
+    L.ph = to_ptrs(*dev, c->d_rng);
+    L.grid = make_grid_view(c);
+    L.P = P;
+    L.queue = c->d_counters + 0;
+    L.n_out = c->d_counters + 1;
+    L.stats = c->d_stats;
+    const int lds_doubles = (c->has_grid && c->separable) ? 2 * (c->nr + c->nth + c->nph) : 0;
+    L.use_lds = lds_doubles > 0 ? 1 : 0;
+    L.lds_doubles = lds_doubles;
+    const size_t lds_bytes = (size_t)lds_doubles * sizeof(double);
+    const int block = c->tune.block;
+    const int persistent_blocks = std::max(1, c->num_cus * c->tune.blocks_per_cu);
+
+    // Global-step schedule, integrate_cart_ks.hpp:468-527.  When the stop
+    // test needs every photon terminated, it cannot end the loop while any
+    // photon would still change, so one epoch of max_steps is equivalent.
+    const int64_t interval = output_interval <= 0 ? 1 : output_interval;
+    const bool exact_schedule = !((double)n * termination_percent > (double)n - 1.0);
+    double fraction = 0.0;
+    int64_t g = 0;
+    int cur = 0;
+    float kernel_ms = 0.0f;
+    uint64_t launches = 0;
+    uint64_t global_steps = 0;
+    while (g < max_steps) {
+        int64_t check;
+        if (!exact_schedule) {
+            check = max_steps - 1;
+        } else {
+            check = ((g + interval - 1) / interval) * interval;
+            if (check > max_steps - 1) check = max_steps - 1;
+        }
+        int64_t left = check - g + 1;
+        int64_t step0 = g;
+        while (left > 0 && live > 0) {
+            const int budget = (int)std::min<int64_t>(left, c->tune.sub_budget);
+            L.list_in = list[cur];
+            L.list_out = list[cur ^ 1];
+            L.n_in = live;
+            L.budget = budget;
+            L.step0 = (int)step0;
+            GRT_HIP(c, hipMemsetAsync(c->d_counters, 0, 2 * sizeof(uint32_t), stream));
+            const int need_blocks = (int)((live + block - 1) / block);
+            const int blocks = std::max(1, std::min(persistent_blocks, need_blocks));
+            GRT_HIP(c, hipEventRecord(c->ev0, stream));
+            GRT_HIP(c, grt::launch_trace(L, blocks, block, lds_bytes, stream));
+            GRT_HIP(c, hipEventRecord(c->ev1, stream));
+            GRT_HIP(c, hipMemcpyAsync(&live, c->d_counters + 1, sizeof(uint32_t),
+                                      hipMemcpyDeviceToHost, stream));
+            GRT_HIP(c, hipStreamSynchronize(stream));
+            float ms = 0.0f;
+            GRT_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            kernel_ms += ms;
+            ++launches;
+            cur ^= 1;
+            left -= budget;
+            step0 += budget;
+        }
+        global_steps = (uint64_t)check + 1;
+        fraction = (double)(n - live) / (double)(n > 0 ? n : 1);
+        g = check + 1;
+        if (fraction >= termination_percent) break;
+        if (!exact_schedule && live == 0) break;
+    }
+    unsigned long long hs[grt::kStatCount];
+    GRT_HIP(c, hipMemcpy(hs, c->d_stats, sizeof hs, hipMemcpyDeviceToHost));
+    if (!exact_schedule && live == 0) {
+        // The reference loop ends at the first output step at or after the
+        // step on which the last photon terminated.
+        const int64_t last = (n > 0 && hs[grt::kStatCalls] > 0) ? (int64_t)hs[grt::kStatLastTermStep] : 0;
+        int64_t chk = ((last + interval - 1) / interval) * interval;
+        if (chk > max_steps - 1) chk = max_steps - 1;
+        global_steps = (uint64_t)chk + 1;
+    }
+    st.functor_calls = hs[grt::kStatCalls];
+    st.accepted_steps = hs[grt::kStatAccepted];
+    st.rejected_attempts = hs[grt::kStatRejected];
+    st.failed_steps = hs[grt::kStatFailed];
+    st.fluid_samples = hs[grt::kStatSamples];
+    st.global_steps = global_steps;
+    st.terminated = n - live;
+    st.launches = launches;
+    st.kernel_ms = kernel_ms;
+    if (stats) *stats = st;
+    return GRT_OK;
+}
+
+int grt_trace(grt_ctx* c, const grt_photons* host, uint64_t n, int32_t max_steps,
+              double termination_percent, int64_t output_interval, grt_stats* stats)
+{
+    if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
+    if (!host || !photons_complete(host)) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
+    GRT_HIP(c, hipSetDevice(c->device));
+    grt_photons d{};
+    int rc = stage_photons(c, n, &d);
+    if (rc != GRT_OK) return rc;
+    rc = copy_photons(c, &d, host, n, hipMemcpyHostToDevice);
+    if (rc != GRT_OK) return rc;
+    rc = grt_trace_device(c, &d, n, max_steps, termination_percent, output_interval, nullptr, stats);
+    if (rc != GRT_OK) return rc;
+    return copy_photons(c, host, &d, n, hipMemcpyDeviceToHost);
+}
+
+namespace {
+
+grt::CameraInit camera_init(const grt_params& p, double initial_step)
+{
+    // initialize_photons.hpp:94-108 and :130-136, :160
+    grt::CameraInit C{};
+    const double d = p.camera_distance, th = p.camera_theta, ph = p.camera_phi;
+    C.c[0] = d * std::sin(th) * std::cos(ph);
+    C.c[1] = d * std::sin(th) * std::sin(ph);
+    C.c[2] = d * std::cos(th);
+    C.t[0] = std::cos(th) * std::cos(ph);
+    C.t[1] = std::cos(th) * std::sin(ph);
+    C.t[2] = -std::sin(th);
+    C.f[0] = -std::sin(ph);
+    C.f[1] = std::cos(ph);
+    C.f[2] = 0.0;
+    double kx = -C.c[0], ky = -C.c[1], kz = -C.c[2];
+    const double kn = std::sqrt(kx * kx + ky * ky + kz * kz);
+    kx /= kn;
+    ky /= kn;
+    kz /= kn;
+    C.k[0] = kx;
+    C.k[1] = ky;
+    C.k[2] = kz;
+    C.dlambda0 = initial_step * d;
+    return C;
+}
+
+}  // namespace
+
+int grt_init_image_device(grt_ctx* c, const grt_photons* dev, uint64_t n, uint64_t id_offset,
+                          double initial_step, const double* screen_u, const double* screen_v,
+                          void* stream_v)
+{
+    if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
+    if (!dev || !photons_complete(dev)) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
+    if ((screen_u == nullptr) != (screen_v == nullptr))
+        return fail(c, GRT_E_INVALID, "screen_u and screen_v must both be given or both be NULL");
+    if (!std::isfinite(initial_step)) return fail(c, GRT_E_INVALID, "initial_step must be finite");
+    GRT_HIP(c, hipSetDevice(c->device));
+    const grt::CameraInit C = camera_init(c->params, initial_step);
+    const PhotonPtrs p = to_ptrs(*dev, nullptr);
+    const grt_params& q = c->params;
+    GRT_HIP(c, grt::launch_init_image(p, n, id_offset, C, q.a_bh, q.m_bh, q.camera_width,
+                                      q.camera_height, q.seed, (q.rng_stream << 1) | 1u, screen_u,
+                                      screen_v, static_cast<hipStream_t>(stream_v)));
+    return GRT_OK;
+}
+
+int grt_init_image(grt_ctx* c, const grt_photons* host, uint64_t n, uint64_t id_offset,
+                   double initial_step, const double* screen_u, const double* screen_v)
+{
+    if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
+    if (!host || !photons_complete(host)) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
+    GRT_HIP(c, hipSetDevice(c->device));
+    grt_photons d{};
+    int rc = stage_photons(c, n, &d);
+    if (rc != GRT_OK) return rc;
+    double* dsu = nullptr;
+    if (screen_u) {
+        GRT_HIP(c, hipMalloc(&dsu, 2 * std::max<uint64_t>(n, 1) * sizeof(double)));
+        hipError_t e = hipMemcpy(dsu, screen_u, n * sizeof(double), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dsu + n, screen_v, n * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            (void)hipFree(dsu);
+            return hip_fail(c, e, "hipMemcpy(screen)");
+        }
+    }
+    rc = grt_init_image_device(c, &d, n, id_offset, initial_step, dsu, dsu ? dsu + n : nullptr,
+                               nullptr);
+    if (rc == GRT_OK) {
+        const hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) rc = hip_fail(c, e, "init_image");
+    }
+    if (dsu) (void)hipFree(dsu);
+    if (rc != GRT_OK) return rc;
+    return copy_photons(c, host, &d, n, hipMemcpyDeviceToHost);
+}
+
+int grt_image_products_device(grt_ctx* c, const grt_photons* dev, uint64_t n, int64_t nx,
+                              int64_t ny, int64_t bins, double smin, double smax, double* image,
+                              double* spec_nu, double* spec_I, uint64_t* arrived, void* stream_v)
+{
+    if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
+    if (!dev || !photons_complete(dev)) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
+    if (nx <= 0 || ny <= 0 || bins <= 0)
+        return fail(c, GRT_E_INVALID, "output image dimensions and spectrum_bins must be positive");
+    if (!(smin > 0.0) || !(smax > smin))
+        return fail(c, GRT_E_INVALID, "output requires 0 < spectrum_min_hz < spectrum_max_hz");
+    if (!image || !spec_nu || !spec_I || !arrived)
+        return fail(c, GRT_E_INVALID, "output pointers must be non-NULL");
+    hipStream_t stream = static_cast<hipStream_t>(stream_v);
+    GRT_HIP(c, hipSetDevice(c->device));
+    // spectrum frequency grid, observation.hpp:48-56
+    const double lmin = std::log10(smin), lmax = std::log10(smax);
+    std::vector<double> nu((size_t)bins);
+    for (int64_t b = 0; b < bins; ++b) {
+        const double fr = bins > 1 ? (double)b / (double)(bins - 1) : 0.0;
+        nu[(size_t)b] = std::pow(10.0, lmin + fr * (lmax - lmin));
+    }
+    GRT_HIP(c, hipMemcpyAsync(spec_nu, nu.data(), nu.size() * sizeof(double),
+                              hipMemcpyHostToDevice, stream));
+    GRT_HIP(c, hipMemsetAsync(image, 0, (size_t)(4 * nx * ny) * sizeof(double), stream));
+    GRT_HIP(c, hipMemsetAsync(spec_I, 0, (size_t)bins * sizeof(double), stream));
+    GRT_HIP(c, hipMemsetAsync(arrived, 0, sizeof(uint64_t), stream));
+    const PhotonPtrs p = to_ptrs(*dev, nullptr);
+    GRT_HIP(c, grt::launch_image_products(p, n, nx, ny, bins, smin, smax, lmin, lmax,
+                                          c->params.camera_width, c->params.camera_height, image,
+                                          spec_I, reinterpret_cast<unsigned long long*>(arrived),
+                                          stream));
+    GRT_HIP(c, hipStreamSynchronize(stream));
+    return GRT_OK;
+}
+
+int grt_image_products(grt_ctx* c, const grt_photons* host, uint64_t n, int64_t nx, int64_t ny,
+                       int64_t bins, double smin, double smax, double* image, double* spec_nu,
+                       double* spec_I, uint64_t* arrived)
+{
+    if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
+    if (!host || !photons_complete(host)) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
+    if (nx <= 0 || ny <= 0 || bins <= 0)
+        return fail(c, GRT_E_INVALID, "output image dimensions and spectrum_bins must be positive");
+    if (!image || !spec_nu || !spec_I || !arrived)
+        return fail(c, GRT_E_INVALID, "output pointers must be non-NULL");
+    GRT_HIP(c, hipSetDevice(c->device));
+    grt_photons d{};
+    int rc = stage_photons(c, n, &d);
+    if (rc != GRT_OK) return rc;
+    rc = copy_photons(c, &d, host, n, hipMemcpyHostToDevice);
+    if (rc != GRT_OK) return rc;
+    const size_t need = (size_t)(4 * nx * ny + 2 * bins + 1);
+    if (need > c->img_cap) {
+        if (c->d_img) (void)hipFree(c->d_img);
+        c->d_img = nullptr;
+        c->img_cap = 0;
+        GRT_HIP(c, hipMalloc(&c->d_img, need * sizeof(double)));
+        c->img_cap = need;
+    }
+    double* dimg = c->d_img;
+    double* dnu = dimg + 4 * nx * ny;
+    double* dsp = dnu + bins;
+    uint64_t* darr = reinterpret_cast<uint64_t*>(dsp + bins);
+    rc = grt_image_products_device(c, &d, n, nx, ny, bins, smin, smax, dimg, dnu, dsp, darr, nullptr);
+    if (rc != GRT_OK) return rc;
+    GRT_HIP(c, hipMemcpy(image, dimg, (size_t)(4 * nx * ny) * sizeof(double), hipMemcpyDeviceToHost));
+    GRT_HIP(c, hipMemcpy(spec_nu, dnu, (size_t)bins * sizeof(double), hipMemcpyDeviceToHost));
+    GRT_HIP(c, hipMemcpy(spec_I, dsp, (size_t)bins * sizeof(double), hipMemcpyDeviceToHost));
+    GRT_HIP(c, hipMemcpy(arrived, darr, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return GRT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,951 @@
+// grt_device.hpp -- per-photon FP64 numerics of the traced path.
+//
+// Every function is __host__ __device__: the tracing kernel inlines them on
+// gfx950, and the CPU harness under tests/ compiles the very same source for
+// the host to cross-check the kernel's logic against the oracle.
+//
+// Arithmetic follows the reference's expressions term by term (same
+// operations, same order; the device compiler may fuse a*b+c into FMA), so the
+// FP64 path agrees with the reference to rounding.  Citations are file:line in
+// the reference's src/ directory.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "grt_extension_points.h"
+
+#define GRT_HD __host__ __device__ __forceinline__
+
+namespace grt {
+
+constexpr double kPi = 3.14159265358979323846;   // utils.hpp:16
+constexpr double kTwoPi = 2.0 * kPi;
+
+GRT_HD bool is_finite(double x) { return __builtin_isfinite(x); }
+
+enum : uint8_t { kDisk = 0, kBackward = 1, kForward = 2, kArrived = 3, kRejected = 4 };
+
+// Constants captured by the step functor (integrate_cart_ks.hpp:35-61), with
+// the camera basis evaluated once on the host.
+struct StepParams {
+    double a, m;
+    double r_min, r_max;
+    double atol, rtol, smin, smax, safety;
+    int integrator, mode;
+    int absorb, emit, scatter;
+    double kabs, kemit, albedo;
+    uint64_t seed;
+    uint32_t stream;
+    int generator;
+    double norm, e_cell, slope, nu_min, nu_max, nu_custom;
+    double cam_d, cam_half_w, cam_half_h;
+    double cam_n[3], cam_t[3], cam_f[3];      // normal, theta_hat, phi_hat
+    double cam_dn[3];                          // camera_distance * normal
+    double length_scale, time_scale, density_scale;
+};
+
+// ---------------------------------------------------------------------------
+// Kerr-Schild geometry, metrics/kerr_schild_core.hpp
+// ---------------------------------------------------------------------------
+
+// kerr_schild_core.hpp:28-33
+GRT_HD double ks_radius(double x, double y, double z, double a)
+{
+    const double rsq = x * x + y * y + z * z;
+    const double A = rsq - a * a;
+    const double az = a * z;
+    const double S = sqrt(A * A + 4.0 * (az * az));
+    return sqrt(0.5 * (A + S));
+}
+
+// kerr_schild_core.hpp:80-83 with l_0 = 1: the g_00 entry, 2 H l_0 l_0 - 1.
+GRT_HD double ks_H(double r, double z, double a, double m)
+{
+    const double r2 = r * r;
+    const double az = a * z;
+    const double D = r2 * r * r + az * az;
+    return m * (r2 * r) / D;
+}
+
+// l_mu of kerr_schild_core.hpp:50-56 (l_0 = 1 implied).
+GRT_HD void ks_l(double x, double y, double z, double r, double a, double l[4])
+{
+    const double B = r * r + a * a;
+    l[0] = 1.0;
+    l[1] = (r * x + a * y) / B;
+    l[2] = (r * y - a * x) / B;
+    l[3] = z / r;
+}
+
+// g_{mu nu} = eta + 2 H l l, kerr_schild_core.hpp:99-110 (row-major 4x4).
+GRT_HD void ks_metric(const double X[4], double a, double m, double g[16])
+{
+    const double r = ks_radius(X[1], X[2], X[3], a);
+    double l[4];
+    ks_l(X[1], X[2], X[3], r, a, l);
+    const double twoH = 2.0 * ks_H(r, X[3], a, m);
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+#pragma unroll
+        for (int nu = 0; nu < 4; ++nu) {
+            const double eta = (mu != nu) ? 0.0 : (mu == 0 ? -1.0 : 1.0);
+            g[4 * mu + nu] = twoH * l[mu] * l[nu] + eta;
+        }
+    }
+}
+
+// Hamilton's equations for H = 1/2 g^{mu nu} p_mu p_nu in Cartesian
+// Kerr-Schild form, kerr_schild_core.hpp:161-192 (with r, dr/dx^k of :36-46,
+// l, dl/dx^k of :59-77 and H, dH/dx^k of :86-95 expanded for k = x, y, z).
+// out[4] (dp_t/dlambda) is identically zero and is not written.
+GRT_HD void ks_rhs(const double s[8], double a, double m, double out[8])
+{
+    const double x = s[1], y = s[2], z = s[3];
+    const double p0 = s[4], p1 = s[5], p2 = s[6], p3 = s[7];
+    const double a2 = a * a;
+
+    const double A = (x * x + y * y + z * z) - a2;
+    const double az = a * z;
+    const double S = sqrt(A * A + 4.0 * (az * az));
+    const double r = sqrt(0.5 * (A + S));
+    const double f = 1.0 + A / S;
+    const double tr = 2.0 * r;
+    const double drx = x * f / tr;
+    const double dry = y * f / tr;
+    const double drz = (z * f + 2.0 * a2 * z / S) / tr;
+
+    const double B = r * r + a2;
+    const double l1 = (r * x + a * y) / B;
+    const double l2 = (r * y - a * x) / B;
+    const double l3 = z / r;
+    const double dBx = tr * drx, dBy = tr * dry, dBz = tr * drz;
+    // dl[k][mu]; the Kronecker terms of :73-75 reduce to +-r, +-a or 1.
+    const double dl_x1 = ((drx * x + r) - l1 * dBx) / B;
+    const double dl_x2 = ((drx * y - a) - l2 * dBx) / B;
+    const double dl_x3 = (-(l3 * drx)) / r;
+    const double dl_y1 = ((dry * x + a) - l1 * dBy) / B;
+    const double dl_y2 = ((dry * y + r) - l2 * dBy) / B;
+    const double dl_y3 = (-(l3 * dry)) / r;
+    const double dl_z1 = (drz * x - l1 * dBz) / B;
+    const double dl_z2 = (drz * y - l2 * dBz) / B;
+    const double dl_z3 = (1.0 - l3 * drz) / r;
+
+    const double r2 = r * r;
+    const double r3 = r2 * r;
+    const double D = r3 * r + az * az;
+    const double H = m * r3 / D;
+    const double D2 = D * D;
+    const double c3 = 3.0 * r2;
+    const double c4 = 4.0 * r3;
+    const double dH_x = m * (c3 * drx * D - r3 * (c4 * drx)) / D2;
+    const double dH_y = m * (c3 * dry * D - r3 * (c4 * dry)) / D2;
+    const double dH_z = m * (c3 * drz * D - r3 * (c4 * drz + 2.0 * a2 * z)) / D2;
+
+    // l^mu = (-1, l1, l2, l3)
+    const double lp = ((-p0 + l1 * p1) + l2 * p2) + l3 * p3;
+    const double twoH = 2.0 * H;
+    out[0] = -p0 + twoH * lp;
+    out[1] = p1 - twoH * l1 * lp;
+    out[2] = p2 - twoH * l2 * lp;
+    out[3] = p3 - twoH * l3 * lp;
+    const double dlp_x = (dl_x1 * p1 + dl_x2 * p2) + dl_x3 * p3;
+    const double dlp_y = (dl_y1 * p1 + dl_y2 * p2) + dl_y3 * p3;
+    const double dlp_z = (dl_z1 * p1 + dl_z2 * p2) + dl_z3 * p3;
+    const double lp2 = lp * lp;
+    const double hl = twoH * lp;
+    out[5] = dH_x * lp2 + hl * dlp_x;
+    out[6] = dH_y * lp2 + hl * dlp_y;
+    out[7] = dH_z * lp2 + hl * dlp_z;
+}
+
+// ---------------------------------------------------------------------------
+// Integrators, integrate_cart_ks.hpp:143-227
+// ---------------------------------------------------------------------------
+
+// Fehlberg 4(5) tableau, integrate_cart_ks.hpp:166-180.
+constexpr double A21 = 1.0 / 4.0;
+constexpr double A31 = 3.0 / 32.0, A32 = 9.0 / 32.0;
+constexpr double A41 = 1932.0 / 2197.0, A42 = -7200.0 / 2197.0, A43 = 7296.0 / 2197.0;
+constexpr double A51 = 439.0 / 216.0, A52 = -8.0, A53 = 3680.0 / 513.0, A54 = -845.0 / 4104.0;
+constexpr double A61 = -8.0 / 27.0, A62 = 2.0, A63 = -3544.0 / 2565.0, A64 = 1859.0 / 4104.0,
+                 A65 = -11.0 / 40.0;
+constexpr double B41 = 25.0 / 216.0, B43 = 1408.0 / 2565.0, B44 = 2197.0 / 4104.0, B45 = -1.0 / 5.0;
+constexpr double B51 = 16.0 / 135.0, B53 = 6656.0 / 12825.0, B54 = 28561.0 / 56430.0,
+                 B55 = -9.0 / 50.0, B56 = 2.0 / 55.0;
+
+// Component 4 (p_t) has a zero derivative, so every stage leaves it equal to
+// s[4]; the loops below skip it.
+#define GRT_FOR_STATE(i) _Pragma("unroll") for (int i = 0; i < 8; ++i) if (i != 4)
+
+// Classic RK4, integrate_cart_ks.hpp:143-155.
+GRT_HD void rk4_step(double s[8], double h, double a, double m)
+{
+    double k1[8], k2[8], k3[8], k4[8], t[8];
+    ks_rhs(s, a, m, k1);
+    t[4] = s[4];
+    GRT_FOR_STATE(i) t[i] = s[i] + 0.5 * h * k1[i];
+    ks_rhs(t, a, m, k2);
+    GRT_FOR_STATE(i) t[i] = s[i] + 0.5 * h * k2[i];
+    ks_rhs(t, a, m, k3);
+    GRT_FOR_STATE(i) t[i] = s[i] + h * k3[i];
+    ks_rhs(t, a, m, k4);
+    const double h6 = h / 6.0;
+    GRT_FOR_STATE(i) s[i] += h6 * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]);
+}
+
+// One RKF45 attempt with precomputed k1, integrate_cart_ks.hpp:160-227.
+// Updates *h to the next proposal; returns true (and replaces s with the
+// 5th-order solution) when accepted.
+//
+// The stage sums are accumulated left to right as each k_j becomes
+// available (P4 = a41 k1 + a42 k2, later P4 + a43 k3, ...), which is exactly
+// the evaluation order of the reference's expressions, so only the partial
+// sums -- not k2..k6 -- stay live in registers.
+GRT_HD bool rk45_attempt(double s[8], double* h_io, const double k1[8], const StepParams& P)
+{
+    const double h = *h_io;
+    double k[8], t[8], P4[8], P5[8], P6[8], X4[8], X5[8];
+    t[4] = s[4];
+    GRT_FOR_STATE(i) t[i] = s[i] + h * A21 * k1[i];
+    ks_rhs(t, P.a, P.m, k);                                   // k2
+    GRT_FOR_STATE(i) {
+        t[i] = s[i] + h * (A31 * k1[i] + A32 * k[i]);
+        P4[i] = A41 * k1[i] + A42 * k[i];
+        P5[i] = A51 * k1[i] + A52 * k[i];
+        P6[i] = A61 * k1[i] + A62 * k[i];
+    }
+    ks_rhs(t, P.a, P.m, k);                                   // k3
+    GRT_FOR_STATE(i) {
+        t[i] = s[i] + h * (P4[i] + A43 * k[i]);
+        P5[i] = P5[i] + A53 * k[i];
+        P6[i] = P6[i] + A63 * k[i];
+        X4[i] = B41 * k1[i] + B43 * k[i];
+        X5[i] = B51 * k1[i] + B53 * k[i];
+    }
+    ks_rhs(t, P.a, P.m, k);                                   // k4
+    GRT_FOR_STATE(i) {
+        t[i] = s[i] + h * (P5[i] + A54 * k[i]);
+        P6[i] = P6[i] + A64 * k[i];
+        X4[i] = X4[i] + B44 * k[i];
+        X5[i] = X5[i] + B54 * k[i];
+    }
+    ks_rhs(t, P.a, P.m, k);                                   // k5
+    GRT_FOR_STATE(i) {
+        t[i] = s[i] + h * (P6[i] + A65 * k[i]);
+        X4[i] = X4[i] + B45 * k[i];
+        X5[i] = X5[i] + B55 * k[i];
+    }
+    ks_rhs(t, P.a, P.m, k);                                   // k6
+
+    double x5[8];
+    double err = 0.0;
+    bool finite = is_finite(s[4]);
+    x5[4] = s[4];
+    GRT_FOR_STATE(i) {
+        const double x4 = s[i] + h * X4[i];
+        x5[i] = s[i] + h * (X5[i] + B56 * k[i]);
+        const double sc = P.atol + P.rtol * fmax(fabs(x5[i]), fabs(x4));
+        const double ratio = fabs(x5[i] - x4) / sc;
+        finite = finite && is_finite(x5[i]) && is_finite(ratio);
+        err = fmax(err, ratio);
+    }
+    if (!finite) {
+        *h_io = h * P.smin;
+        return false;
+    }
+    double scale;
+    if (err == 0.0) {
+        scale = P.smax;
+    } else {
+        scale = P.safety * pow(1.0 / err, 0.25);
+        scale = fmin(P.smax, fmax(P.smin, scale));
+    }
+    *h_io = h * scale;
+    if (err <= 1.0) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) s[i] = x5[i];
+        return true;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// Fluid grid and nearest-cell search, radiative_transfer/scattering.hpp:41-251
+// ---------------------------------------------------------------------------
+
+// Cell record, built on the host from the .npy fields: density and
+// temperature times their unit scales, and the fluid four-velocity and
+// magnetic four-vector already mapped to Cartesian Kerr-Schild components
+// (fluid_frame.hpp:29-49) and scaled (scattering.hpp:212-241).
+constexpr int kCellDoubles = 10;
+
+struct GridView {
+    int nr, nth, nph;
+    int available;
+    int separable;            // 1: coordinates depend only on their own index
+    // separable fast path: 1-D axes and 0.500001 * neighbour span per index
+    const double* r_axis;
+    const double* r_thr;
+    const double* th_axis;
+    const double* th_thr;
+    const double* ph_axis;
+    const double* ph_thr;
+    // general path: the full (nr, nth, nph) coordinate fields
+    const double* r3;
+    const double* th3;
+    const double* ph3;
+    const double* cells;      // kCellDoubles per cell, C order
+};
+
+// scattering.hpp:41-45
+GRT_HD double periodic_distance(double lhs, double rhs)
+{
+    const double d = fmod(fabs(lhs - rhs), kTwoPi);
+    return fmin(d, kTwoPi - d);
+}
+
+// Number of entries <= v in the ascending table t[0..n).
+GRT_HD int count_le(const double* t, int n, double v)
+{
+    int lo = 0, len = n;
+    while (len > 0) {
+        const int half = len >> 1;
+        if (t[lo + half] <= v) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    return lo;
+}
+
+// Separable fast path of locate_structured_grid_cell (scattering.hpp:151-180).
+// With r(i,j,k) = r(i), theta(i,j,k) = theta(j), phi(i,j,k) = phi(k), the
+// three coordinate-descent passes converge in the first, and every
+// nearest_* scan (first strict minimum of the same distance expression) is
+// resolved by a binary search plus the bracketing candidates: |log(r/c)| and
+// |theta - c| are V-shaped in the index, and the periodic azimuth distance is
+// minimised at the bracket or at either end of the axis.  The host only
+// enables this path for positive, strictly ascending axes with phi in [0, 2pi).
+GRT_HD bool locate_separable(const GridView& G, double radius, double polar, double az,
+                             int& i_out, int& j_out, int& k_out)
+{
+    // radial (scattering.hpp:47-65)
+    int i;
+    double di;
+    {
+        const int lo = count_le(G.r_axis, G.nr, radius) - 1;
+        if (lo < 0) {
+            i = 0;
+            di = fabs(log(radius / G.r_axis[0]));
+        } else if (lo >= G.nr - 1) {
+            i = G.nr - 1;
+            di = fabs(log(radius / G.r_axis[G.nr - 1]));
+        } else {
+            const double d0 = fabs(log(radius / G.r_axis[lo]));
+            const double d1 = fabs(log(radius / G.r_axis[lo + 1]));
+            i = (d1 < d0) ? lo + 1 : lo;
+            di = (d1 < d0) ? d1 : d0;
+        }
+    }
+    // polar (scattering.hpp:67-81)
+    int j;
+    double dj;
+    {
+        const int lo = count_le(G.th_axis, G.nth, polar) - 1;
+        if (lo < 0) {
+            j = 0;
+            dj = fabs(polar - G.th_axis[0]);
+        } else if (lo >= G.nth - 1) {
+            j = G.nth - 1;
+            dj = fabs(polar - G.th_axis[G.nth - 1]);
+        } else {
+            const double d0 = fabs(polar - G.th_axis[lo]);
+            const double d1 = fabs(polar - G.th_axis[lo + 1]);
+            j = (d1 < d0) ? lo + 1 : lo;
+            dj = (d1 < d0) ? d1 : d0;
+        }
+    }
+    // azimuth (scattering.hpp:83-97): candidates 0, lo, lo+1, n-1 in index order
+    int k = 0;
+    double dk = periodic_distance(az, G.ph_axis[0]);
+    {
+        const int n = G.nph;
+        const int lo = count_le(G.ph_axis, n, az) - 1;
+        const int c1 = lo, c2 = lo + 1, c3 = n - 1;
+        if (c1 > 0 && c1 < n) {
+            const double d = periodic_distance(az, G.ph_axis[c1]);
+            if (d < dk) { dk = d; k = c1; }
+        }
+        if (c2 > 0 && c2 > c1 && c2 < n) {
+            const double d = periodic_distance(az, G.ph_axis[c2]);
+            if (d < dk) { dk = d; k = c2; }
+        }
+        if (c3 > 0 && c3 > c2) {
+            const double d = periodic_distance(az, G.ph_axis[c3]);
+            if (d < dk) { dk = d; k = c3; }
+        }
+    }
+    i_out = i;
+    j_out = j;
+    k_out = k;
+    // within_* tests (scattering.hpp:99-149); thr = 0.500001 * neighbour span,
+    // or -1 for a single-entry axis whose test is distance <= 1e-10.
+    const double ti = G.r_thr[i], tj = G.th_thr[j], tk = G.ph_thr[k];
+    const bool wi = (ti < 0.0) ? (di <= 1.0e-10) : (di <= ti);
+    const bool wj = (tj < 0.0) ? (dj <= 1.0e-10) : (dj <= tj);
+    const bool wk = (tk < 0.0) ? (dk <= 1.0e-10) : (dk <= tk);
+    return wi && wj && wk;
+}
+
+GRT_HD double c3at(const double* a, const GridView& G, int i, int j, int k)
+{
+    return a[((size_t)i * G.nth + j) * G.nph + k];
+}
+
+// General path: the reference's three coordinate-descent passes of linear
+// scans over arbitrary 3-D coordinate fields (scattering.hpp:47-180).
+GRT_HD bool locate_general(const GridView& G, double radius, double polar, double az,
+                           int& i_out, int& j_out, int& k_out)
+{
+    int i = G.nr / 2, j = G.nth / 2, k = G.nph / 2;
+    for (int it = 0; it < 3; ++it) {
+        {
+            const double first = c3at(G.r3, G, 0, j, k);
+            if (!(radius > 0.0) || !(first > 0.0)) return false;
+            int best = 0;
+            double bd = fabs(log(radius / first));
+            for (int q = 1; q < G.nr; ++q) {
+                const double c = c3at(G.r3, G, q, j, k);
+                if (!(c > 0.0)) continue;
+                const double d = fabs(log(radius / c));
+                if (d < bd) { best = q; bd = d; }
+            }
+            i = best;
+        }
+        {
+            int best = 0;
+            double bd = fabs(polar - c3at(G.th3, G, i, 0, k));
+            for (int q = 1; q < G.nth; ++q) {
+                const double d = fabs(polar - c3at(G.th3, G, i, q, k));
+                if (d < bd) { best = q; bd = d; }
+            }
+            j = best;
+        }
+        {
+            int best = 0;
+            double bd = periodic_distance(az, c3at(G.ph3, G, i, j, 0));
+            for (int q = 1; q < G.nph; ++q) {
+                const double d = periodic_distance(az, c3at(G.ph3, G, i, j, q));
+                if (d < bd) { best = q; bd = d; }
+            }
+            k = best;
+        }
+    }
+    i_out = i;
+    j_out = j;
+    k_out = k;
+    // within_radial_cell, scattering.hpp:99-117
+    {
+        const double c = c3at(G.r3, G, i, j, k);
+        if (!(c > 0.0) || !(radius > 0.0)) return false;
+        const double cd = fabs(log(radius / c));
+        if (G.nr == 1) {
+            if (!(cd <= 1.0e-10)) return false;
+        } else {
+            double span = 0.0;
+            if (i > 0 && c3at(G.r3, G, i - 1, j, k) > 0.0)
+                span = fmax(span, fabs(log(c / c3at(G.r3, G, i - 1, j, k))));
+            if (i + 1 < G.nr && c3at(G.r3, G, i + 1, j, k) > 0.0)
+                span = fmax(span, fabs(log(c / c3at(G.r3, G, i + 1, j, k))));
+            if (!(cd <= 0.500001 * span)) return false;
+        }
+    }
+    // within_polar_cell, scattering.hpp:119-133
+    {
+        const double c = c3at(G.th3, G, i, j, k);
+        const double cd = fabs(polar - c);
+        if (G.nth == 1) {
+            if (!(cd <= 1.0e-10)) return false;
+        } else {
+            double span = 0.0;
+            if (j > 0) span = fmax(span, fabs(c - c3at(G.th3, G, i, j - 1, k)));
+            if (j + 1 < G.nth) span = fmax(span, fabs(c - c3at(G.th3, G, i, j + 1, k)));
+            if (!(cd <= 0.500001 * span)) return false;
+        }
+    }
+    // within_azimuth_cell, scattering.hpp:135-149
+    {
+        const double c = c3at(G.ph3, G, i, j, k);
+        const double cd = periodic_distance(az, c);
+        if (G.nph == 1) return cd <= 1.0e-10;
+        double span = 0.0;
+        if (k > 0) span = fmax(span, periodic_distance(c, c3at(G.ph3, G, i, j, k - 1)));
+        if (k + 1 < G.nph) span = fmax(span, periodic_distance(c, c3at(G.ph3, G, i, j, k + 1)));
+        return cd <= 0.500001 * span;
+    }
+}
+
+// The Kerr-Schild metric at one point in factored form,
+// g_{mu nu} = eta_{mu nu} + twoH l_mu l_nu with l_0 = 1
+// (kerr_schild_core.hpp:99-110).
+struct KSMetric {
+    double twoH, l1, l2, l3;
+};
+
+GRT_HD KSMetric ks_metric_at(const double s[8], double a, double m)
+{
+    const double r = ks_radius(s[1], s[2], s[3], a);
+    double l[4];
+    ks_l(s[1], s[2], s[3], r, a, l);
+    return KSMetric{2.0 * ks_H(r, s[3], a, m), l[1], l[2], l[3]};
+}
+
+// metric_inner_product (fluid_frame.hpp:6-14) through the factored metric:
+// eta(u, v) + twoH (l.u)(l.v).  Equal to the 16-term sum up to rounding.
+GRT_HD double inner(const KSMetric& g, const double u[4], const double v[4])
+{
+    const double lu = u[0] + g.l1 * u[1] + g.l2 * u[2] + g.l3 * u[3];
+    const double lv = v[0] + g.l1 * v[1] + g.l2 * v[2] + g.l3 * v[3];
+    return (-u[0] * v[0] + u[1] * v[1] + u[2] * v[2] + u[3] * v[3]) + g.twoH * lu * lv;
+}
+
+// build_fluid_tetrad, fluid_frame.hpp:52-80: e[0] = u / |u|, then
+// Gram-Schmidt of the coordinate x, y, z axes under g.
+GRT_HD bool fluid_tetrad(const KSMetric& g, const double u[4], double e[16])
+{
+    const double vn = inner(g, u, u);
+    if (!(vn < 0.0) || !is_finite(vn)) return false;
+    const double inv = 1.0 / sqrt(-vn);
+    const double sgn = u[0] >= 0.0 ? 1.0 : -1.0;
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) e[mu] = sgn * u[mu] * inv;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        double c[4] = {0.0, 0.0, 0.0, 0.0};
+        c[ax + 1] = 1.0;
+        const double tp = inner(g, e, c);
+#pragma unroll
+        for (int mu = 0; mu < 4; ++mu) c[mu] += tp * e[mu];
+#pragma unroll
+        for (int pv = 0; pv < ax; ++pv) {
+            const double pr = inner(g, e + 4 * (pv + 1), c);
+#pragma unroll
+            for (int mu = 0; mu < 4; ++mu) c[mu] -= pr * e[4 * (pv + 1) + mu];
+        }
+        const double nn = inner(g, c, c);
+        if (!(nn > 1.0e-20) || !is_finite(nn)) return false;
+        const double is = 1.0 / sqrt(nn);
+#pragma unroll
+        for (int mu = 0; mu < 4; ++mu) e[4 * (ax + 1) + mu] = c[mu] * is;
+    }
+    return true;
+}
+
+// sample_scattering_fluid, scattering.hpp:185-251.  `g` receives the metric at
+// the photon position (reused by the scattering update).
+GRT_HD bool sample_fluid(const double s[8], const GridView& G, const StepParams& P,
+                         FluidCellCGS& fl, double e[16], KSMetric& g)
+{
+    if (!G.available) return false;
+    const double radius = ks_radius(s[1], s[2], s[3], P.a);
+    if (!(radius > 0.0) || !is_finite(radius)) return false;
+    const double ct = fmax(-1.0, fmin(1.0, s[3] / radius));
+    const double polar = acos(ct);
+    double az = atan2(radius * s[2] - P.a * s[1], radius * s[1] + P.a * s[2]);
+    if (az < 0.0) az += kTwoPi;
+    int i, j, k;
+    const bool found = G.separable ? locate_separable(G, radius, polar, az, i, j, k)
+                                   : locate_general(G, radius, polar, az, i, j, k);
+    if (!found) return false;
+    const double* c = G.cells + (((size_t)i * G.nth + j) * G.nph + k) * kCellDoubles;
+    fl.position_cm[0] = s[1] * P.length_scale;
+    fl.position_cm[1] = s[2] * P.length_scale;
+    fl.position_cm[2] = s[3] * P.length_scale;
+    fl.coordinate_time_s = s[0] * P.time_scale;
+    fl.density_g_cm3 = c[0];
+    fl.temperature_k = c[1];
+    if (!(fl.density_g_cm3 > 0.0) || !is_finite(fl.density_g_cm3) || !is_finite(fl.temperature_k))
+        return false;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) fl.coordinate_four_velocity_cm_s[q] = c[2 + q];
+    g = ks_metric_at(s, P.a, P.m);
+    if (!fluid_tetrad(g, fl.coordinate_four_velocity_cm_s, e)) return false;
+    const double b[4] = {c[6], c[7], c[8], c[9]};
+    fl.magnetic_gauss[0] = -inner(g, e, b);
+    fl.magnetic_gauss[1] = inner(g, e + 4, b);
+    fl.magnetic_gauss[2] = inner(g, e + 8, b);
+    fl.magnetic_gauss[3] = inner(g, e + 12, b);
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Counter-based RNG: Philox4x32-10 (Salmon, Moraes, Dror & Shaw, SC'11).
+// Replaces the Kokkos XorShift64 pool (scattering.hpp:16-17).
+// ---------------------------------------------------------------------------
+
+GRT_HD void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)p1;
+        c[3] = (uint32_t)p0;
+        c[0] = n0;
+        c[2] = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// Per-photon stream: counter (draw, stream, id_lo, id_hi), key = seed.
+struct PhotonRng {
+    uint64_t id;
+    uint64_t seed;
+    uint32_t stream;
+    uint32_t draw;
+    GRT_HD double drand()
+    {
+        uint32_t c[4] = {draw, stream, (uint32_t)id, (uint32_t)(id >> 32)};
+        philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+        draw += 1u;
+        const uint64_t mant = ((uint64_t)(c[0] >> 5) << 26) | (uint64_t)(c[1] >> 6);
+        return (double)mant * (1.0 / 9007199254740992.0);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Emission, scattering and camera
+// ---------------------------------------------------------------------------
+
+// GenerateBlackbody / GeneratePowerLaw (photon_generation.hpp:99-143), the
+// GeneratePhotonsCustom plug-in, selected as integrate_cart_ks.hpp:332-347.
+GRT_HD bool generate_photon(const StepParams& P, const FluidCellCGS& cell, PhotonRng& rng,
+                            int packets, double& nu, double& energy)
+{
+    if (P.generator == 0) {
+        if (!(cell.temperature_k > 0.0)) return false;
+        const double inverse_zeta4 = 0.9239384029215902;
+        const double choose = rng.drand();
+        double cum = 0.0;
+        int idx = 1;
+        for (; idx < 64; ++idx) {
+            const double n = (double)idx;
+            cum += inverse_zeta4 / (n * n * n * n);
+            if (choose <= cum) break;
+        }
+        const double d1 = rng.drand();
+        const double d2 = rng.drand();
+        const double d3 = rng.drand();
+        const double d4 = rng.drand();
+        const double prod = fmax(1.0e-300, d1 * d2 * d3 * d4);
+        const double xx = -log(prod) / (double)idx;
+        nu = xx * 2.0836619123327576e10 * cell.temperature_k;
+        energy = P.e_cell / (double)packets;
+        return is_finite(nu) && nu > 0.0;
+    }
+    if (P.generator == 1) {
+        const double draw = rng.drand();
+        if (fabs(P.slope - 1.0) < 1.0e-12) {
+            nu = P.nu_min * pow(P.nu_max / P.nu_min, draw);
+        } else {
+            const double ex = 1.0 - P.slope;
+            const double lo = pow(P.nu_min, ex);
+            const double hi = pow(P.nu_max, ex);
+            nu = pow(lo + draw * (hi - lo), 1.0 / ex);
+        }
+        energy = P.e_cell / (double)packets;
+        return is_finite(nu) && nu > 0.0;
+    }
+    return GeneratePhotonsCustom(cell, rng, packets, P.nu_custom, P.e_cell, nu, energy);
+}
+
+constexpr double kProtonMassCgs = 1.67262192369e-24;    // scattering.hpp:256
+constexpr double kSigmaThomsonCgs = 6.6524587158e-25;   // scattering.hpp:257
+
+// maybe_scatter_photon, scattering.hpp:259-313 (g = metric at the photon).
+GRT_HD bool maybe_scatter(double s[8], double stokes[4], const StepParams& P, double dl_abs,
+                          const FluidCellCGS& fl, const double e[16], const KSMetric& g,
+                          PhotonRng& rng)
+{
+    const double path = dl_abs * P.length_scale;
+    const double ne = fl.density_g_cm3 / kProtonMassCgs;
+    const double tau = kSigmaThomsonCgs * ne * path;
+    const double prob = 1.0 - exp(-tau);
+    if (rng.drand() >= prob) return false;
+    double eloc = 0.0;
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) eloc -= s[4 + mu] * e[mu];
+    if (!(eloc > 0.0) || !is_finite(eloc)) return false;
+    const double mu = 2.0 * rng.drand() - 1.0;
+    const double dphi = kTwoPi * rng.drand();
+    const double tr = sqrt(fmax(0.0, 1.0 - mu * mu));
+    const double dx = tr * cos(dphi), dy = tr * sin(dphi), dz = mu;
+    // fluid_frame_direction_to_covariant_momentum, fluid_frame.hpp:83-104:
+    // p_mu = g_{mu nu} w^nu = eta_{mu nu} w^nu + twoH l_mu (l.w)
+    double w[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) w[q] = e[q] + dx * e[4 + q] + dy * e[8 + q] + dz * e[12 + q];
+    const double lw = w[0] + g.l1 * w[1] + g.l2 * w[2] + g.l3 * w[3];
+    s[4] = eloc * (-w[0] + g.twoH * lw);
+    s[5] = eloc * (w[1] + g.twoH * g.l1 * lw);
+    s[6] = eloc * (w[2] + g.twoH * g.l2 * lw);
+    s[7] = eloc * (w[3] + g.twoH * g.l3 * lw);
+    stokes[0] *= P.albedo;
+    stokes[1] *= 0.5 * P.albedo;
+    stokes[2] *= 0.5 * P.albedo;
+    stokes[3] *= 0.25 * P.albedo;
+    return true;
+}
+
+// camera_plane_crossing, integrate_cart_ks.hpp:96-120
+GRT_HD int camera_crossing(const StepParams& P, const double s[8], double& u, double& v)
+{
+    const double plane = s[1] * P.cam_n[0] + s[2] * P.cam_n[1] + s[3] * P.cam_n[2];
+    if (plane < P.cam_d) return 0;
+    const double r0 = s[1] - P.cam_dn[0], r1 = s[2] - P.cam_dn[1], r2 = s[3] - P.cam_dn[2];
+    u = r0 * P.cam_t[0] + r1 * P.cam_t[1] + r2 * P.cam_t[2];
+    v = r0 * P.cam_f[0] + r1 * P.cam_f[1] + r2 * P.cam_f[2];
+    return (fabs(u) <= P.cam_half_w && fabs(v) <= P.cam_half_h) ? 1 : -1;
+}
+
+// camera_observer_energy, integrate_cart_ks.hpp:122-130 (g_00 = 2H - 1).
+GRT_HD double observer_energy(const StepParams& P, const double s[8])
+{
+    const double r = ks_radius(s[1], s[2], s[3], P.a);
+    const double g00 = 2.0 * ks_H(r, s[3], P.a, P.m) + (-1.0);
+    if (!(g00 < 0.0)) return 0.0;
+    return -s[4] * (1.0 / sqrt(-g00));
+}
+
+// ---------------------------------------------------------------------------
+// Per-photon state and the post-step part of the functor
+// ---------------------------------------------------------------------------
+
+struct Photon {
+    double s[8];
+    double stokes[4];
+    double freq, eem, dl;
+    uint8_t phase, term;
+};
+
+// Everything after the RK step in Geodesic_cartesian_kerr_schild::operator()
+// (integrate_cart_ks.hpp:308-404): fluid sampling, the image-mode emission
+// event, radiative transfer, scattering and the camera test.  `phase0` is the
+// phase the call started with; `taken` the affine step just integrated.
+// On arrival at the camera (phase becomes kArrived) the screen coordinates
+// are returned in (au, av).  Returns whether the fluid was sampled.
+GRT_HD bool post_step(Photon& ph, uint8_t phase0, double taken, const GridView& G,
+                      const StepParams& P, PhotonRng& rng, double& au, double& av)
+{
+    const bool image = (P.mode == 0);
+    const bool needs = P.scatter || P.absorb || P.emit || (image && phase0 == kBackward);
+    FluidCellCGS fl;
+    double e[16];
+    KSMetric g;
+    const bool sampled = needs && sample_fluid(ph.s, G, P, fl, e, g);
+
+    if (image && phase0 == kBackward && sampled) {
+        double nu = 0.0, energy = 0.0;
+        bool generated = false;
+        const double dcode = fl.density_g_cm3 / P.density_scale;
+        const int packets = ComputeSuperphotonsPerCell(dcode, P.norm);
+        if (packets > 0) generated = generate_photon(P, fl, rng, packets, nu, energy);
+        if (generated) {
+            ph.freq = nu;
+            double eem = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) eem -= ph.s[4 + q] * e[q];
+            ph.eem = eem;
+            ph.stokes[0] = energy;
+            ph.stokes[1] = ph.stokes[2] = ph.stokes[3] = 0.0;
+            ph.dl = -fabs(ph.dl);
+            ph.phase = kForward;
+        } else {
+            ph.stokes[0] = ph.stokes[1] = ph.stokes[2] = ph.stokes[3] = 0.0;
+            ph.term = 1;
+            ph.phase = kRejected;
+        }
+    } else if (sampled && phase0 != kBackward) {
+        // apply_radiative_transfer_step, radiative_transfer.hpp:81-96
+        if (P.absorb) {
+            const double att = exp(-ComputeOpticalDepthIncrement(fl, ph.freq, taken, P.kabs));
+#pragma unroll
+            for (int q = 0; q < 4; ++q) ph.stokes[q] *= att;
+        }
+        if (P.emit) ph.stokes[0] += ComputeEmissivityIncrement(fl, ph.freq, taken, P.kemit);
+        if (P.scatter) maybe_scatter(ph.s, ph.stokes, P, fabs(ph.dl), fl, e, g, rng);
+    }
+
+    if (image && phase0 == kForward) {
+        const int crossing = camera_crossing(P, ph.s, au, av);
+        if (crossing > 0) {
+            const double eobs = observer_energy(P, ph.s);
+            const double eem = ph.eem;
+            if (eobs > 0.0 && eem > 0.0 && is_finite(eobs) && is_finite(eem))
+                ph.freq *= eobs / eem;
+            else
+                ph.stokes[0] = ph.stokes[1] = ph.stokes[2] = ph.stokes[3] = 0.0;
+            ph.phase = kArrived;
+            ph.term = 1;
+        } else if (crossing < 0) {
+            ph.stokes[0] = ph.stokes[1] = ph.stokes[2] = ph.stokes[3] = 0.0;
+            ph.phase = kRejected;
+            ph.term = 1;
+        }
+    }
+    return needs;
+}
+
+// The radius tests that open every functor call (integrate_cart_ks.hpp:248-263).
+// Returns true when the photon terminated.
+GRT_HD bool radius_exit(Photon& ph, const StepParams& P)
+{
+    const double dist = ks_radius(ph.s[1], ph.s[2], ph.s[3], P.a);
+    if (dist <= P.r_min) {
+        ph.stokes[0] = ph.stokes[1] = ph.stokes[2] = ph.stokes[3] = 0.0;
+        ph.term = 1;
+        ph.phase = kRejected;
+        return true;
+    }
+    if (dist > P.r_max) {
+        ph.term = 1;
+        ph.phase = kRejected;
+        return true;
+    }
+    return false;
+}
+
+// null_covariant_momentum_from_spatial_direction, kerr_schild_core.hpp:201-227:
+// the future-directed K^0 making (K^0, K) null, lowered to p_mu.
+GRT_HD bool null_momentum(const double X[4], const double K[3], double a, double m, double p[4])
+{
+    double g[16];
+    ks_metric(X, a, m, g);
+    const double qa = g[0];
+    const double qb = 2.0 * (g[1] * K[0] + g[2] * K[1] + g[3] * K[2]);
+    double qc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) qc += g[4 * (i + 1) + (j + 1)] * K[i] * K[j];
+    const double disc = qb * qb - 4.0 * qa * qc;
+    if (disc < 0.0) return false;
+    const double sq = sqrt(disc);
+    const double r1 = (-qb + sq) / (2.0 * qa);
+    const double r2 = (-qb - sq) / (2.0 * qa);
+    const double K0 = (r1 > 0.0) ? r1 : r2;
+    if (!(K0 > 0.0)) return false;
+    const double Kc[4] = {K0, K[0], K[1], K[2]};
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+        double s = 0.0;
+#pragma unroll
+        for (int nu = 0; nu < 4; ++nu) s += g[4 * mu + nu] * Kc[nu];
+        p[mu] = s;
+    }
+    return true;
+}
+
+// Camera frame of initialize_photons_image (initialize_photons.hpp:94-108),
+// evaluated on the host.
+struct CameraInit {
+    double c[3];          // plane centre d (sin th cos ph, sin th sin ph, cos th)
+    double t[3], f[3];    // theta_hat, phi_hat
+    double k[3];          // -c / |c|
+    double dlambda0;      // initial_step * camera_distance
+};
+
+// One image-mode ray at screen (u, v), initialize_photons.hpp:123-166.
+GRT_HD void init_image_ray(const CameraInit& C, double u, double v, double a, double m,
+                           double s[8])
+{
+    const double px = C.c[0] + u * C.t[0] + v * C.f[0];
+    const double py = C.c[1] + u * C.t[1] + v * C.f[1];
+    const double pz = C.c[2] + u * C.t[2] + v * C.f[2];
+    const double X[4] = {0.0, px, py, pz};
+    double p[4] = {0.0, 0.0, 0.0, 0.0};
+    null_momentum(X, C.k, a, m, p);
+    s[0] = 0.0;
+    s[1] = px;
+    s[2] = py;
+    s[3] = pz;
+    s[4] = p[0];
+    s[5] = p[1];
+    s[6] = p[2];
+    s[7] = p[3];
+}
+
+// Work counters of one lane.
+struct LaneCounters {
+    uint32_t calls, accepted, rejected, failed, samples;
+};
+
+// One photon resident in one lane between refills.
+struct Lane {
+    Photon ph;
+    double k1[8];
+    int attempt;
+    bool in_call;
+    uint8_t phase0;
+};
+
+GRT_HD void lane_reset(Lane& L)
+{
+    L.in_call = false;
+    L.attempt = 0;
+    L.phase0 = 0;
+}
+
+// One iteration of a lane: at most one RK45 attempt (or one RK4 step) of the
+// functor call in progress, opening a new call first if none is in progress.
+// Geodesic_cartesian_kerr_schild::operator() (integrate_cart_ks.hpp:231-419)
+// is thereby split at its retry loop, so lanes whose step is rejected retry
+// while their neighbours go on.  Returns true when the functor call
+// completed in this iteration; (au, av) are set when the photon arrived.
+template <int kIntegrator>
+GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, PhotonRng& rng,
+                         LaneCounters& n, double& au, double& av)
+{
+    Photon& ph = L.ph;
+    if (!L.in_call) {
+        L.phase0 = ph.phase;
+        if (radius_exit(ph, P)) return true;
+        if (kIntegrator != 1) {
+            // RK4, integrate_cart_ks.hpp:269-272
+            const double taken = ph.dl;
+            rk4_step(ph.s, taken, P.a, P.m);
+            ++n.accepted;
+            n.samples += post_step(ph, L.phase0, taken, G, P, rng, au, av) ? 1u : 0u;
+            return true;
+        }
+        ks_rhs(ph.s, P.a, P.m, L.k1);
+        L.attempt = 0;
+        L.in_call = true;
+    }
+    // One attempt of the retry loop, integrate_cart_ks.hpp:273-299.
+    const double taken = ph.dl;
+    const bool accepted = rk45_attempt(ph.s, &ph.dl, L.k1, P);
+    ++L.attempt;
+    if (accepted) {
+        ++n.accepted;
+        L.in_call = false;
+        n.samples += post_step(ph, L.phase0, taken, G, P, rng, au, av) ? 1u : 0u;
+        return true;
+    }
+    ++n.rejected;
+    if (L.attempt == 12) {
+        ++n.failed;
+        ph.term = 1;
+        L.in_call = false;
+        return true;
+    }
+    return false;
+}
+
+}  // namespace grt

@@ -1,0 +1,230 @@
+// grt_host_prep.hpp -- host-side preparation of what the tracing kernel
+// consumes: the functor constants and the fluid-grid tables / cell records.
+// Pure C++ (no HIP runtime calls); shared by the library and the CPU test
+// harness.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <thread>
+#include <vector>
+
+#include "grt.h"
+#include "grt_device.hpp"
+
+namespace grt {
+
+// Functor constants (integrate_cart_ks.hpp:471-476) with the camera basis of
+// integrate_cart_ks.hpp:98-107 evaluated once here.
+inline StepParams make_step_params(const grt_params& p)
+{
+    StepParams s{};
+    s.a = p.a_bh;
+    s.m = p.m_bh;
+    s.r_min = p.r_term_min;
+    s.r_max = p.r_term_max;
+    s.atol = p.atol;
+    s.rtol = p.rtol;
+    s.smin = p.min_step_scale;
+    s.smax = p.max_step_scale;
+    s.safety = p.safety;
+    s.integrator = p.integrator;
+    s.mode = p.mode;
+    s.absorb = p.absorption_enabled ? 1 : 0;
+    s.emit = p.emission_enabled ? 1 : 0;
+    s.scatter = p.scattering_enabled ? 1 : 0;
+    s.kabs = p.absorption_coefficient;
+    s.kemit = p.emission_coefficient;
+    s.albedo = p.albedo;
+    s.seed = p.seed;
+    s.stream = p.rng_stream;
+    s.generator = p.generator;
+    s.norm = p.superphoton_count_normalization;
+    s.e_cell = p.energy_per_cell_erg;
+    s.slope = p.power_law_slope;
+    s.nu_min = p.nu_min_hz;
+    s.nu_max = p.nu_max_hz;
+    s.nu_custom = p.custom_frequency_hz;
+    const double th = p.camera_theta, ph = p.camera_phi, d = p.camera_distance;
+    s.cam_d = d;
+    s.cam_half_w = 0.5 * p.camera_width;
+    s.cam_half_h = 0.5 * p.camera_height;
+    s.cam_n[0] = std::sin(th) * std::cos(ph);
+    s.cam_n[1] = std::sin(th) * std::sin(ph);
+    s.cam_n[2] = std::cos(th);
+    s.cam_t[0] = std::cos(th) * std::cos(ph);
+    s.cam_t[1] = std::cos(th) * std::sin(ph);
+    s.cam_t[2] = -std::sin(th);
+    s.cam_f[0] = -std::sin(ph);
+    s.cam_f[1] = std::cos(ph);
+    s.cam_f[2] = 0.0;
+    for (int q = 0; q < 3; ++q) s.cam_dn[q] = d * s.cam_n[q];
+    s.length_scale = p.length_cm_per_code;
+    s.time_scale = p.time_s_per_code;
+    s.density_scale = p.density_g_cm3_per_code;
+    return s;
+}
+
+template <typename F>
+void host_parallel_for(size_t n, F&& f)
+{
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 65536) nt = 1;
+    std::vector<std::thread> th;
+    const size_t chunk = (n + nt - 1) / nt;
+    for (unsigned t = 0; t < nt; ++t) {
+        const size_t lo = t * chunk, hi = std::min(n, lo + chunk);
+        if (lo >= hi) break;
+        th.emplace_back([&f, lo, hi] {
+            for (size_t i = lo; i < hi; ++i) f(i);
+        });
+    }
+    for (auto& x : th) x.join();
+}
+
+// Everything the kernel reads about the grid, still in host memory.
+struct HostGrid {
+    int nr = 0, nth = 0, nph = 0;
+    bool separable = false;
+    std::vector<double> tables;   // r_axis, r_thr, th_axis, th_thr, ph_axis, ph_thr
+    std::vector<double> cells;    // kCellDoubles per cell
+};
+
+// Largest separable table set staged in LDS (bytes).
+constexpr size_t kMaxTableBytes = 48 * 1024;
+
+inline void prepare_grid(const grt_params& p, int nr, int nth, int nph, const double* r,
+                         const double* theta, const double* phi, const double* rho,
+                         const double* tgas, const double* vel, const double* mag, HostGrid& out)
+{
+    const size_t G = (size_t)nr * nth * nph;
+    out.nr = nr;
+    out.nth = nth;
+    out.nph = nph;
+    auto at = [&](const double* a, int i, int j, int k) {
+        return a[((size_t)i * nth + j) * nph + k];
+    };
+    // Separable axes: r(i,j,k) = r(i), theta = theta(j), phi = phi(k), r > 0,
+    // strictly ascending, phi in [0, 2 pi), tables small enough for LDS.
+    const double two_pi = 2.0 * 3.14159265358979323846;
+    bool sep = true;
+    for (size_t c = 0; c < G && sep; ++c) {
+        const int i = (int)(c / ((size_t)nth * nph));
+        const int j = (int)((c / nph) % nth);
+        const int k = (int)(c % nph);
+        if (r[c] != at(r, i, 0, 0) || theta[c] != at(theta, 0, j, 0) || phi[c] != at(phi, 0, 0, k))
+            sep = false;
+    }
+    std::vector<double> ra(nr), ta(nth), pa(nph);
+    for (int i = 0; i < nr; ++i) ra[i] = at(r, i, 0, 0);
+    for (int j = 0; j < nth; ++j) ta[j] = at(theta, 0, j, 0);
+    for (int k = 0; k < nph; ++k) pa[k] = at(phi, 0, 0, k);
+    for (int i = 0; i < nr && sep; ++i)
+        if (!(ra[i] > 0.0) || !std::isfinite(ra[i]) || (i > 0 && !(ra[i] > ra[i - 1]))) sep = false;
+    for (int j = 0; j < nth && sep; ++j)
+        if (!std::isfinite(ta[j]) || (j > 0 && !(ta[j] > ta[j - 1]))) sep = false;
+    for (int k = 0; k < nph && sep; ++k)
+        if (!(pa[k] >= 0.0) || !(pa[k] < two_pi) || (k > 0 && !(pa[k] > pa[k - 1]))) sep = false;
+    if ((size_t)2 * (nr + nth + nph) * sizeof(double) > kMaxTableBytes) sep = false;
+    out.separable = sep;
+    out.tables.clear();
+    if (sep) {
+        // within_* thresholds, scattering.hpp:99-149: 0.500001 * neighbour
+        // span; -1 marks a single-entry axis (test distance <= 1e-10 instead).
+        out.tables.assign(2 * (size_t)(nr + nth + nph), 0.0);
+        double* rt = out.tables.data();
+        double* rth = rt + nr;
+        double* tt = rt + 2 * nr;
+        double* tth = tt + nth;
+        double* pt = tt + 2 * nth;
+        double* pth = pt + nph;
+        for (int i = 0; i < nr; ++i) {
+            rt[i] = ra[i];
+            double span = 0.0;
+            if (i > 0 && ra[i - 1] > 0.0) span = std::fmax(span, std::fabs(std::log(ra[i] / ra[i - 1])));
+            if (i + 1 < nr && ra[i + 1] > 0.0)
+                span = std::fmax(span, std::fabs(std::log(ra[i] / ra[i + 1])));
+            rth[i] = nr == 1 ? -1.0 : 0.500001 * span;
+        }
+        for (int j = 0; j < nth; ++j) {
+            tt[j] = ta[j];
+            double span = 0.0;
+            if (j > 0) span = std::fmax(span, std::fabs(ta[j] - ta[j - 1]));
+            if (j + 1 < nth) span = std::fmax(span, std::fabs(ta[j] - ta[j + 1]));
+            tth[j] = nth == 1 ? -1.0 : 0.500001 * span;
+        }
+        auto per = [&](double a, double b) {
+            const double d = std::fmod(std::fabs(a - b), two_pi);
+            return std::fmin(d, two_pi - d);
+        };
+        for (int k = 0; k < nph; ++k) {
+            pt[k] = pa[k];
+            double span = 0.0;
+            if (k > 0) span = std::fmax(span, per(pa[k], pa[k - 1]));
+            if (k + 1 < nph) span = std::fmax(span, per(pa[k], pa[k + 1]));
+            pth[k] = nph == 1 ? -1.0 : 0.500001 * span;
+        }
+    }
+    // Cell records: scaled density and temperature, and the Cartesian
+    // Kerr-Schild four-velocity / magnetic vector of each cell
+    // (transform_spherical_four_vector_to_cartesian, fluid_frame.hpp:29-49,
+    // evaluated at the cell's own r, theta, phi as scattering.hpp:223-241 does).
+    out.cells.assign(G * kCellDoubles, 0.0);
+    const double spin = p.a_bh;
+    const double ds = p.density_g_cm3_per_code, ts = p.temperature_k_per_code;
+    const double vs = p.four_velocity_cm_s_per_code, ms = p.magnetic_gauss_per_code;
+    double* cells = out.cells.data();
+    host_parallel_for(G, [&](size_t c) {
+        const double rc = r[c], thc = theta[c], phc = phi[c];
+        const double st = std::sin(thc), ct = std::cos(thc), sp = std::sin(phc), cp = std::cos(phc);
+        const double rx = cp * st, ry = sp * st, rz = ct;
+        const double tx = (rc * cp - spin * sp) * ct;
+        const double ty = (rc * sp + spin * cp) * ct;
+        const double tz = -rc * st;
+        const double px = (-rc * sp - spin * cp) * st;
+        const double py = (rc * cp - spin * sp) * st;
+        double* o = cells + c * kCellDoubles;
+        const double* v = vel + 4 * c;
+        const double* b = mag + 4 * c;
+        o[0] = rho[c] * ds;
+        o[1] = tgas[c] * ts;
+        o[2] = v[0] * vs;
+        o[3] = (rx * v[1] + tx * v[2] + px * v[3]) * vs;
+        o[4] = (ry * v[1] + ty * v[2] + py * v[3]) * vs;
+        o[5] = (rz * v[1] + tz * v[2]) * vs;
+        o[6] = b[0] * ms;
+        o[7] = (rx * b[1] + tx * b[2] + px * b[3]) * ms;
+        o[8] = (ry * b[1] + ty * b[2] + py * b[3]) * ms;
+        o[9] = (rz * b[1] + tz * b[2]) * ms;
+    });
+}
+
+// GridView over host (or device) copies of HostGrid's arrays.
+inline GridView grid_view(int nr, int nth, int nph, bool available, bool separable,
+                          const double* tables, const double* coords3, const double* cells)
+{
+    GridView g{};
+    g.nr = nr;
+    g.nth = nth;
+    g.nph = nph;
+    g.available = available ? 1 : 0;
+    g.separable = separable ? 1 : 0;
+    if (available && separable) {
+        g.r_axis = tables;
+        g.r_thr = tables + nr;
+        g.th_axis = tables + 2 * nr;
+        g.th_thr = tables + 2 * nr + nth;
+        g.ph_axis = tables + 2 * nr + 2 * nth;
+        g.ph_thr = tables + 2 * nr + 2 * nth + nph;
+    }
+    if (available && !separable) {
+        const size_t G = (size_t)nr * nth * nph;
+        g.r3 = coords3;
+        g.th3 = coords3 + G;
+        g.ph3 = coords3 + 2 * G;
+    }
+    g.cells = cells;
+    return g;
+}
+
+}  // namespace grt

@@ -1,0 +1,71 @@
+// grt_internal.hpp -- interface between the host driver (grt_api.cpp) and
+// the HIP kernels (grt_kernels.hip).  Not part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "grt.h"
+#include "grt_device.hpp"
+
+namespace grt {
+
+// Device copy of the photon SoA (utils.hpp:40-54) plus the per-photon RNG
+// draw counter the library keeps alongside it.
+struct PhotonPtrs {
+    uint64_t* id;
+    double* freq;
+    double* eem;
+    double* x[4];
+    double* k[4];
+    double* st[4];
+    double* dl;
+    uint8_t* term;
+    uint8_t* phase;
+    double* udisp;
+    double* vdisp;
+    uint32_t* rng_draw;
+};
+
+enum StatSlot {
+    kStatCalls = 0,
+    kStatAccepted,
+    kStatRejected,
+    kStatFailed,
+    kStatSamples,
+    kStatLastTermStep,   // max over photons of the global step on which they terminated
+    kStatCount
+};
+
+struct TraceLaunch {
+    PhotonPtrs ph;
+    GridView grid;
+    StepParams P;
+    const uint32_t* list_in;
+    uint32_t n_in;
+    uint32_t* queue;          // work counter over list_in, zeroed before launch
+    uint32_t* list_out;       // photons still live after `budget` calls
+    uint32_t* n_out;          // zeroed before launch
+    unsigned long long* stats;  // kStatCount slots
+    int budget;               // functor calls per photon in this launch
+    int step0;                // global step index of the first call of this launch
+    int use_lds;              // stage the separable search tables in LDS
+    int lds_doubles;          // size of that staging area
+};
+
+// Kernel launchers (grt_kernels.hip).  They only enqueue work on `stream`.
+hipError_t launch_trace(const TraceLaunch& a, int blocks, int block, size_t lds_bytes,
+                        hipStream_t stream);
+hipError_t launch_build_live(const uint8_t* term, uint64_t n, uint32_t* list, uint32_t* count,
+                             hipStream_t stream);
+hipError_t launch_image_products(const PhotonPtrs& ph, uint64_t n, int64_t nx, int64_t ny,
+                                 int64_t bins, double smin, double smax, double lmin, double lmax,
+                                 double width, double height, double* image, double* spec,
+                                 unsigned long long* arrived, hipStream_t stream);
+hipError_t launch_init_image(const PhotonPtrs& ph, uint64_t n, uint64_t id0, const CameraInit& C,
+                             double a, double m, double w, double h, uint64_t seed,
+                             uint32_t stream, const double* su, const double* sv,
+                             hipStream_t hs);
+hipError_t trace_kernel_attributes(int* vgprs, int* sgprs, int* lds_static, int* max_threads);
+
+}  // namespace grt

@@ -1,0 +1,330 @@
+"""Python host binding of libgrt_amd.so (the HIP tracing path, C ABI in include/grt.h).
+
+This mirrors the reference's host interface for the traced path:
+`Context(params)` stands for the Geodesic_cartesian_kerr_schild functor's
+captured constants (integrate_cart_ks.hpp:63-94), `set_grid` for the
+.npy field bundle (load_python_arrays.hpp:195-223), `trace` for
+integrate_geodesics (integrate_cart_ks.hpp:423-535) and `image_products`
+for build_observation_products (observation.hpp:31-84).  Photons are the
+20-field structure of arrays of utils.hpp:40-54, given as a dict of numpy
+arrays (host) or of torch tensors on the GPU (device, no copies).
+
+Every call goes to the HIP library; there is no CPU fallback.  Loading fails
+loudly if libgrt_amd.so has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_ROOT, "libgrt_amd.so")
+
+GRT_RK4, GRT_RK45 = 0, 1
+MODE_IMAGE, MODE_DISK = 0, 1
+GEN_BLACKBODY, GEN_POWER_LAW, GEN_CUSTOM = 0, 1, 2
+PHASE_DISK, PHASE_BACKWARD, PHASE_FORWARD, PHASE_ARRIVED, PHASE_REJECTED = range(5)
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+
+
+class GrtError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("a_bh", C.c_double), ("m_bh", C.c_double),
+        ("r_term_min", C.c_double), ("r_term_max", C.c_double),
+        ("atol", C.c_double), ("rtol", C.c_double),
+        ("min_step_scale", C.c_double), ("max_step_scale", C.c_double),
+        ("safety", C.c_double),
+        ("integrator", C.c_int32), ("mode", C.c_int32),
+        ("absorption_enabled", C.c_int32), ("emission_enabled", C.c_int32),
+        ("absorption_coefficient", C.c_double), ("emission_coefficient", C.c_double),
+        ("scattering_enabled", C.c_int32), ("albedo", C.c_double),
+        ("seed", C.c_uint64), ("rng_stream", C.c_uint32),
+        ("generator", C.c_int32),
+        ("superphoton_count_normalization", C.c_double),
+        ("energy_per_cell_erg", C.c_double), ("power_law_slope", C.c_double),
+        ("nu_min_hz", C.c_double), ("nu_max_hz", C.c_double),
+        ("custom_frequency_hz", C.c_double),
+        ("camera_distance", C.c_double), ("camera_theta", C.c_double),
+        ("camera_phi", C.c_double), ("camera_width", C.c_double),
+        ("camera_height", C.c_double),
+        ("length_cm_per_code", C.c_double), ("time_s_per_code", C.c_double),
+        ("density_g_cm3_per_code", C.c_double), ("temperature_k_per_code", C.c_double),
+        ("four_velocity_cm_s_per_code", C.c_double), ("magnetic_gauss_per_code", C.c_double),
+        ("precision", C.c_int32),
+    ]
+
+
+PHOTON_FIELDS = [
+    ("id", np.uint64), ("frequency", np.float64), ("emission_frame_energy", np.float64),
+    ("x0", np.float64), ("x1", np.float64), ("x2", np.float64), ("x3", np.float64),
+    ("k0", np.float64), ("k1", np.float64), ("k2", np.float64), ("k3", np.float64),
+    ("I", np.float64), ("Q", np.float64), ("U", np.float64), ("V", np.float64),
+    ("dlambda", np.float64), ("terminate", np.uint8), ("phase", np.uint8),
+    ("theta_disp", np.float64), ("phi_disp", np.float64),
+]
+
+
+class Photons(C.Structure):
+    _fields_ = [(name, C.c_void_p) for name, _ in PHOTON_FIELDS]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "functor_calls", "accepted_steps", "rejected_attempts", "failed_steps",
+        "fluid_samples", "global_steps", "terminated", "launches")] + [("kernel_ms", C.c_double)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("sub_budget", C.c_int32)]
+
+
+EXPORTED = [
+    "grt_params_default", "grt_create", "grt_destroy", "grt_set_params", "grt_set_grid",
+    "grt_trace", "grt_trace_device", "grt_image_products", "grt_image_products_device",
+    "grt_init_image", "grt_init_image_device",
+    "grt_get_tuning", "grt_set_tuning", "grt_last_error", "grt_abi_version",
+]
+
+_lib = None
+
+
+def load():
+    """Load libgrt_amd.so (raises GrtError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GrtError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                       "or python gr-ray-tracer_amd/grt_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    P, PH, ST, TU = C.POINTER(Params), C.POINTER(Photons), C.POINTER(Stats), C.POINTER(Tuning)
+    vp, dp = C.c_void_p, C.POINTER(C.c_double)
+    sig = {
+        "grt_params_default": (None, [P]),
+        "grt_create": (C.c_int, [P, C.POINTER(vp)]),
+        "grt_destroy": (C.c_int, [vp]),
+        "grt_set_params": (C.c_int, [vp, P]),
+        "grt_set_grid": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32] + [vp] * 7),
+        "grt_trace": (C.c_int, [vp, PH, C.c_uint64, C.c_int32, C.c_double, C.c_int64, ST]),
+        "grt_trace_device": (C.c_int, [vp, PH, C.c_uint64, C.c_int32, C.c_double, C.c_int64,
+                                       vp, ST]),
+        "grt_image_products": (C.c_int, [vp, PH, C.c_uint64, C.c_int64, C.c_int64, C.c_int64,
+                                         C.c_double, C.c_double, vp, vp, vp, vp]),
+        "grt_image_products_device": (C.c_int, [vp, PH, C.c_uint64, C.c_int64, C.c_int64,
+                                                C.c_int64, C.c_double, C.c_double, vp, vp, vp,
+                                                vp, vp]),
+        "grt_init_image": (C.c_int, [vp, PH, C.c_uint64, C.c_uint64, C.c_double, vp, vp]),
+        "grt_init_image_device": (C.c_int, [vp, PH, C.c_uint64, C.c_uint64, C.c_double, vp, vp,
+                                            vp]),
+        "grt_get_tuning": (C.c_int, [vp, TU]),
+        "grt_set_tuning": (C.c_int, [vp, TU]),
+        "grt_last_error": (C.c_char_p, [vp]),
+        "grt_abi_version": (C.c_int, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def default_params(**overrides):
+    """grt_params filled with the reference defaults, then `overrides`."""
+    p = Params()
+    load().grt_params_default(C.byref(p))
+    for k, v in overrides.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def empty_photons(n):
+    return {name: np.zeros(n, dtype=dt) for name, dt in PHOTON_FIELDS}
+
+
+def _ptr(a):
+    if hasattr(a, "data_ptr"):               # torch tensor
+        if not a.is_contiguous():
+            raise GrtError("photon tensors must be contiguous")
+        return a.data_ptr()
+    if not a.flags.c_contiguous:
+        raise GrtError("photon arrays must be C-contiguous")
+    return a.ctypes.data
+
+
+def photon_struct(ph):
+    s = Photons()
+    for name, dt in PHOTON_FIELDS:
+        a = ph[name]
+        if not hasattr(a, "data_ptr") and a.dtype != dt:
+            raise GrtError(f"photon field {name} must be {np.dtype(dt).name}")
+        setattr(s, name, _ptr(a))
+    return s
+
+
+def _stats_dict(st):
+    return {name: getattr(st, name) for name, _ in Stats._fields_}
+
+
+class Context:
+    """A tracing context on the current HIP device (grt_create)."""
+
+    def __init__(self, params):
+        self._L = load()
+        h = C.c_void_p()
+        rc = self._L.grt_create(C.byref(params), C.byref(h))
+        if rc != 0:
+            raise GrtError(self._L.grt_last_error(None).decode())
+        self._h = h
+        self.params = params
+        self._keep = None
+
+    def _check(self, rc):
+        if rc != 0:
+            raise GrtError(self._L.grt_last_error(self._h).decode())
+
+    def close(self):
+        if self._h:
+            self._L.grt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params):
+        self._check(self._L.grt_set_params(self._h, C.byref(params)))
+        self.params = params
+
+    def set_grid(self, fields):
+        """fields: dict with r, theta, phi, rho, Tgas (nr, nth, nph) and vel, mag (.., 4)."""
+        arr = {k: np.ascontiguousarray(fields[k], dtype=np.float64)
+               for k in ("r", "theta", "phi", "rho", "Tgas", "vel", "mag")}
+        nr, nth, nph = arr["r"].shape
+        for k in ("theta", "phi", "rho", "Tgas"):
+            if arr[k].shape != (nr, nth, nph):
+                raise GrtError(f"Grid field has incompatible dimensions: {k}")
+        for k in ("vel", "mag"):
+            if arr[k].shape != (nr, nth, nph, 4):
+                raise GrtError(f"Grid field has incompatible dimensions: {k}")
+        self._check(self._L.grt_set_grid(
+            self._h, nr, nth, nph, *[arr[k].ctypes.data for k in
+                                     ("r", "theta", "phi", "rho", "Tgas", "vel", "mag")]))
+
+    def trace(self, photons, max_steps, termination_percent=1.0, output_interval=0,
+              stream=None):
+        """Advance photons in place.  numpy dict -> host entry point; torch
+        tensors on the GPU -> device entry point on `stream` (torch stream or
+        raw handle, default: torch's current stream)."""
+        n = len(photons["x0"])
+        st = Stats()
+        s = photon_struct(photons)
+        if hasattr(photons["x0"], "data_ptr"):
+            handle = _stream_handle(stream)
+            self._check(self._L.grt_trace_device(self._h, C.byref(s), n, max_steps,
+                                                 termination_percent, output_interval,
+                                                 handle, C.byref(st)))
+        else:
+            self._check(self._L.grt_trace(self._h, C.byref(s), n, max_steps,
+                                          termination_percent, output_interval, C.byref(st)))
+        return _stats_dict(st)
+
+    def init_image(self, photons, initial_step, screen_u=None, screen_v=None, id_offset=0,
+                   stream=None):
+        """Fill `photons` (numpy dict or GPU tensor dict) with image-mode rays."""
+        n = len(photons["x0"])
+        s = photon_struct(photons)
+        if hasattr(photons["x0"], "data_ptr"):
+            su = screen_u.data_ptr() if screen_u is not None else None
+            sv = screen_v.data_ptr() if screen_v is not None else None
+            self._check(self._L.grt_init_image_device(self._h, C.byref(s), n, id_offset,
+                                                      initial_step, su, sv,
+                                                      _stream_handle(stream)))
+        else:
+            su = sv = None
+            if screen_u is not None:
+                screen_u = np.ascontiguousarray(screen_u, dtype=np.float64)
+                screen_v = np.ascontiguousarray(screen_v, dtype=np.float64)
+                su, sv = screen_u.ctypes.data, screen_v.ctypes.data
+            self._check(self._L.grt_init_image(self._h, C.byref(s), n, id_offset, initial_step,
+                                               su, sv))
+        return photons
+
+    def image_products(self, photons, nx, ny, spectrum_bins=128, spectrum_min_hz=1e8,
+                       spectrum_max_hz=1e22):
+        n = len(photons["x0"])
+        s = photon_struct(photons)
+        if hasattr(photons["x0"], "data_ptr"):
+            import torch
+            dev = photons["x0"].device
+            img = torch.empty(4 * nx * ny, dtype=torch.float64, device=dev)
+            nu = torch.empty(spectrum_bins, dtype=torch.float64, device=dev)
+            sp = torch.empty(spectrum_bins, dtype=torch.float64, device=dev)
+            arr = torch.zeros(1, dtype=torch.int64, device=dev)
+            self._check(self._L.grt_image_products_device(
+                self._h, C.byref(s), n, nx, ny, spectrum_bins, spectrum_min_hz,
+                spectrum_max_hz, img.data_ptr(), nu.data_ptr(), sp.data_ptr(), arr.data_ptr(),
+                _stream_handle(None)))
+            return dict(image=img.view(4, ny, nx), spectrum_frequency_hz=nu, spectrum_I=sp,
+                        arrived_count=int(arr.item()))
+        img = np.zeros(4 * nx * ny)
+        nu = np.zeros(spectrum_bins)
+        sp = np.zeros(spectrum_bins)
+        arr = np.zeros(1, dtype=np.uint64)
+        self._check(self._L.grt_image_products(
+            self._h, C.byref(s), n, nx, ny, spectrum_bins, spectrum_min_hz, spectrum_max_hz,
+            img.ctypes.data, nu.ctypes.data, sp.ctypes.data, arr.ctypes.data))
+        return dict(image=img.reshape(4, ny, nx), spectrum_frequency_hz=nu, spectrum_I=sp,
+                    arrived_count=int(arr[0]))
+
+    @property
+    def tuning(self):
+        t = Tuning()
+        self._check(self._L.grt_get_tuning(self._h, C.byref(t)))
+        return dict(block=t.block, blocks_per_cu=t.blocks_per_cu, sub_budget=t.sub_budget)
+
+    @tuning.setter
+    def tuning(self, d):
+        t = Tuning()
+        self._check(self._L.grt_get_tuning(self._h, C.byref(t)))
+        for k, v in d.items():
+            setattr(t, k, v)
+        self._check(self._L.grt_set_tuning(self._h, C.byref(t)))
+
+
+def _stream_handle(stream):
+    if stream is None:
+        import torch
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if hasattr(stream, "cuda_stream"):
+        return C.c_void_p(stream.cuda_stream)
+    return C.c_void_p(stream)
+
+
+def to_device(photons, device="cuda:0"):
+    """Copy a numpy photon dict to torch tensors on the GPU."""
+    import torch
+    out = {}
+    for name, dt in PHOTON_FIELDS:
+        a = np.ascontiguousarray(photons[name])
+        if dt == np.uint64:
+            a = a.view(np.int64)
+        out[name] = torch.from_numpy(a.copy()).to(device)
+    return out
+
+
+def to_host(photons):
+    """Copy torch photon tensors back to a numpy dict."""
+    out = {}
+    for name, dt in PHOTON_FIELDS:
+        a = photons[name].detach().cpu().numpy()
+        out[name] = a.view(dt) if dt == np.uint64 else a.astype(dt, copy=False)
+    return out

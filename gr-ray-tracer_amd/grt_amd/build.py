@@ -1,0 +1,71 @@
+"""Compile libgrt_amd.so (the HIP tracing kernels + the C ABI) for gfx950.
+
+The library is built in-tree next to this package so it travels with the
+repository to the GPU host.  Packed FP32 instructions are disabled for the
+device code (`-packed-fp32-ops`); the host pass warns that it does not know
+that feature, which is harmless.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG_DIR)                 # gr-ray-tracer_amd/
+REPO = os.path.dirname(ROOT)
+CSRC = os.path.join(ROOT, "csrc")
+INCLUDE = os.path.join(REPO, "include")
+LIB_PATH = os.path.join(ROOT, "libgrt_amd.so")
+
+SOURCES = ["grt_kernels.hip", "grt_api.cpp"]
+HEADERS = ["grt_device.hpp", "grt_internal.hpp"]
+
+
+def hipcc():
+    for cand in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found: the HIP toolchain is required to build libgrt_amd.so")
+
+
+def compile_flags(resource_usage=False):
+    flags = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-function",
+             "-I" + INCLUDE, "-I" + CSRC]
+    if resource_usage:
+        flags.append("-Rpass-analysis=kernel-resource-usage")
+    return flags
+
+
+def build(verbose=False, resource_usage=False):
+    """Compile every source into libgrt_amd.so (always recompiles)."""
+    cc = hipcc()
+    objs = []
+    build_dir = os.path.join(ROOT, "_build")
+    os.makedirs(build_dir, exist_ok=True)
+    for src in SOURCES:
+        obj = os.path.join(build_dir, src + ".o")
+        cmd = [cc] + compile_flags(resource_usage and src.endswith(".hip")) + \
+            ["-c", os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"hipcc failed on {src}")
+        if verbose or resource_usage:
+            sys.stderr.write(r.stderr)
+        objs.append(obj)
+    tmp = LIB_PATH + ".tmp"
+    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc link of libgrt_amd.so failed")
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+if __name__ == "__main__":
+    build(verbose=True, resource_usage="--usage" in sys.argv)

@@ -1,0 +1,198 @@
+/*
+ * grt.h -- C ABI of the MI355X-native GR-ray-tracer hot path (libgrt_amd.so).
+ *
+ * One context owns the device copy of the fluid grid and the scratch state of
+ * the persistent tracing kernel.  Every entry point returns 0 on success and a
+ * negative GRT_E* code on failure; grt_last_error() then returns the message.
+ * No exception crosses this boundary.  Calls on one context are serialised by
+ * the caller; separate contexts may be used from separate host threads.
+ *
+ * Per-photon failures stay in-band exactly as in the reference
+ * (terminate / phase / zeroed Stokes), see integrate_cart_ks.hpp:231-419.
+ */
+#ifndef GRT_H
+#define GRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GRT_ABI_VERSION 1
+
+enum {
+    GRT_OK = 0,
+    GRT_E_INVALID = -1,   /* bad argument, shape or configuration */
+    GRT_E_NOGRID = -2,    /* a fluid-dependent feature with no grid set */
+    GRT_E_DEVICE = -3,    /* HIP runtime error */
+    GRT_E_NOMEM = -4      /* device allocation failed */
+};
+
+enum { GRT_RK4 = 0, GRT_RK45 = 1 };                 /* utils.hpp:169 IntegratorType */
+enum { GRT_MODE_IMAGE = 0, GRT_MODE_DISK = 1 };     /* simulation_options.hpp:11 SimulationMode */
+enum { GRT_GEN_BLACKBODY = 0, GRT_GEN_POWER_LAW = 1, GRT_GEN_CUSTOM = 2 }; /* photon_generation.hpp:12-16 */
+enum {                                              /* utils.hpp:135-141 PhotonPhase */
+    GRT_PHASE_DISK = 0, GRT_PHASE_IMAGE_BACKWARD = 1, GRT_PHASE_IMAGE_FORWARD = 2,
+    GRT_PHASE_IMAGE_ARRIVED = 3, GRT_PHASE_REJECTED = 4
+};
+enum { GRT_PRECISION_FP64 = 0, GRT_PRECISION_MIXED = 1 };
+
+/*
+ * Everything Geodesic_cartesian_kerr_schild captures at construction
+ * (integrate_cart_ks.hpp:35-61, :63-94) plus UnitConversions
+ * (photon_generation.hpp:18-25).  Fill with grt_params_default() first.
+ */
+typedef struct {
+    double a_bh, m_bh;                        /* spin, mass (utils.cpp:10-11) */
+    double r_term_min, r_term_max;            /* termination radii (utils.hpp:164-167) */
+    double atol, rtol;                        /* RK45 tolerances (utils.cpp:18-19) */
+    double min_step_scale, max_step_scale, safety; /* utils.hpp:178-180 */
+    int32_t integrator;                       /* GRT_RK4 / GRT_RK45 */
+    int32_t mode;                             /* GRT_MODE_IMAGE / GRT_MODE_DISK */
+    int32_t absorption_enabled, emission_enabled;   /* RadiativeTransferModel */
+    double absorption_coefficient, emission_coefficient;
+    int32_t scattering_enabled;               /* ScatteringModel (scattering.hpp:10-14) */
+    double albedo;
+    uint64_t seed;                            /* RNG key */
+    uint32_t rng_stream;                      /* RNG stream selector, see grt_trace */
+    int32_t generator;                        /* PhotonGenerationConfig */
+    double superphoton_count_normalization, energy_per_cell_erg, power_law_slope;
+    double nu_min_hz, nu_max_hz, custom_frequency_hz;
+    double camera_distance, camera_theta, camera_phi, camera_width, camera_height;
+    double length_cm_per_code, time_s_per_code, density_g_cm3_per_code;
+    double temperature_k_per_code, four_velocity_cm_s_per_code, magnetic_gauss_per_code;
+    int32_t precision;                        /* GRT_PRECISION_* */
+} grt_params;
+
+/*
+ * The Photons structure of arrays, utils.hpp:40-54 (same fields, same types;
+ * bool is one byte).  For grt_trace these are host pointers, for
+ * grt_trace_device device pointers.  Every field must be non-NULL.
+ */
+typedef struct {
+    uint64_t *id;
+    double *frequency, *emission_frame_energy;
+    double *x0, *x1, *x2, *x3;
+    double *k0, *k1, *k2, *k3;                /* covariant p_mu */
+    double *I, *Q, *U, *V, *dlambda;
+    uint8_t *terminate, *phase;
+    double *theta_disp, *phi_disp;
+} grt_photons;
+
+typedef struct {
+    uint64_t functor_calls;      /* per-photon step calls on live photons */
+    uint64_t accepted_steps;     /* accepted RK45 steps (or RK4 steps) */
+    uint64_t rejected_attempts;  /* rejected RK45 attempts */
+    uint64_t failed_steps;       /* steps that hit the 12-attempt limit */
+    uint64_t fluid_samples;      /* sample_scattering_fluid calls */
+    uint64_t global_steps;       /* host-loop iterations the reference would run */
+    uint64_t terminated;         /* photons with terminate set at return */
+    uint64_t launches;           /* tracing-kernel launches */
+    double kernel_ms;            /* summed duration of the tracing-kernel launches */
+} grt_stats;
+
+/* Launch shape of the persistent tracing kernel (chosen by the autotuner). */
+typedef struct {
+    int32_t block;           /* work-items per workgroup (64..1024, multiple of 64) */
+    int32_t blocks_per_cu;   /* resident workgroups requested per CU */
+    int32_t sub_budget;      /* max steps per photon per launch before compaction */
+} grt_tuning;
+
+/* Fills *p with the reference's compiled-in defaults (utils.cpp:4-37,
+ * simulation_options.hpp:13-32, photon_generation.hpp:40-51). */
+void grt_params_default(grt_params *p);
+
+typedef struct grt_ctx grt_ctx;
+
+/* Replaces Geodesic_cartesian_kerr_schild's constructor
+ * (integrate_cart_ks.hpp:63-94) together with the parameter validation of
+ * toml_config.hpp:238-277 that concerns the traced path.  Creates a context
+ * on the current HIP device. */
+int grt_create(const grt_params *p, grt_ctx **out);
+int grt_destroy(grt_ctx *ctx);
+
+/* Replaces the functor's captured constants for later calls
+ * (integrate_cart_ks.hpp:471-476 rebuilds the functor each step). */
+int grt_set_params(grt_ctx *ctx, const grt_params *p);
+
+/*
+ * Replaces load_numpy_field_bundle (load_python_arrays.hpp:195-223) followed by
+ * the ScatteringFluidGrid construction (integrate_cart_ks.hpp:453-459).
+ * Host arrays in the .npy layout: r, theta, phi, rho, tgas are C-order
+ * (nr, nth, nph) float64; vel and mag are (nr, nth, nph, 4).  The unit scales
+ * of the current params are folded in here, so call again after a
+ * grt_set_params that changes them.
+ */
+int grt_set_grid(grt_ctx *ctx, int32_t nr, int32_t nth, int32_t nph,
+                 const double *r, const double *theta, const double *phi,
+                 const double *rho, const double *tgas, const double *vel, const double *mag);
+
+/*
+ * Replaces integrate_geodesics (integrate_cart_ks.hpp:423-535) without its
+ * output archives: advances every photon with the reference's global-step
+ * semantics -- at most max_steps per-photon steps, the terminated fraction
+ * re-counted on steps where step % output_interval == 0 or
+ * step == max_steps - 1 (output_interval <= 0 counts as 1), stopping once it
+ * reaches termination_percent.  Photons are updated in place.
+ * Random draws (scattering, image-mode emission) come from a Philox4x32-10
+ * stream keyed by (seed, photon id, rng_stream, draw number); the draw number
+ * restarts at 0 on every call, so change rng_stream between calls that must
+ * not reuse draws.
+ * grt_trace takes host arrays (copied in and out); grt_trace_device takes
+ * device arrays and runs on `stream` (a hipStream_t, NULL = default stream).
+ * stats may be NULL.
+ */
+int grt_trace(grt_ctx *ctx, const grt_photons *host, uint64_t n, int32_t max_steps,
+              double termination_percent, int64_t output_interval, grt_stats *stats);
+int grt_trace_device(grt_ctx *ctx, const grt_photons *dev, uint64_t n, int32_t max_steps,
+                     double termination_percent, int64_t output_interval, void *stream,
+                     grt_stats *stats);
+
+/*
+ * Replaces initialize_photons_image (initialize_photons.hpp:82-170): fills n
+ * image-mode rays on the camera plane of the current params with
+ * id = id_offset + i, the null covariant momentum toward the hole,
+ * dlambda = initial_step * camera_distance and phase IMAGE_BACKWARD.  Screen
+ * positions (theta_disp, phi_disp) are taken from screen_u/screen_v, or, when
+ * both are NULL, drawn uniformly over the plane from the photon's
+ * initialisation stream (Philox, stream (rng_stream << 1) | 1) in place of the
+ * reference's XorShift64 pool.  Host arrays for grt_init_image, device
+ * arrays for grt_init_image_device.
+ */
+int grt_init_image(grt_ctx *ctx, const grt_photons *host, uint64_t n, uint64_t id_offset,
+                   double initial_step, const double *screen_u, const double *screen_v);
+int grt_init_image_device(grt_ctx *ctx, const grt_photons *dev, uint64_t n, uint64_t id_offset,
+                          double initial_step, const double *screen_u, const double *screen_v,
+                          void *stream);
+
+/*
+ * Replaces build_observation_products (observation.hpp:31-84): sums Stokes of
+ * photons with phase == GRT_PHASE_IMAGE_ARRIVED into image[4][ny][nx]
+ * (I, Q, U, V; pixel = y * nx + x, x from theta_disp, y from phi_disp, the
+ * bin rule of observation.hpp:22-29) and spectrum_I[bins] on the log10
+ * frequency grid spectrum_nu[bins].  Outputs are host arrays for
+ * grt_image_products and device arrays for grt_image_products_device.
+ */
+int grt_image_products(grt_ctx *ctx, const grt_photons *host, uint64_t n, int64_t nx, int64_t ny,
+                       int64_t spectrum_bins, double spectrum_min_hz, double spectrum_max_hz,
+                       double *image, double *spectrum_nu, double *spectrum_I, uint64_t *arrived);
+int grt_image_products_device(grt_ctx *ctx, const grt_photons *dev, uint64_t n, int64_t nx,
+                              int64_t ny, int64_t spectrum_bins, double spectrum_min_hz,
+                              double spectrum_max_hz, double *image, double *spectrum_nu,
+                              double *spectrum_I, uint64_t *arrived, void *stream);
+
+/* Launch-shape control of the tracing kernel (no reference counterpart). */
+int grt_get_tuning(grt_ctx *ctx, grt_tuning *t);
+int grt_set_tuning(grt_ctx *ctx, const grt_tuning *t);
+
+/* Message of the last failing call on ctx (or of the last failing
+ * grt_create when ctx is NULL). */
+const char *grt_last_error(const grt_ctx *ctx);
+int grt_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

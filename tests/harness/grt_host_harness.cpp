@@ -1,0 +1,127 @@
+// grt_host_harness.cpp -- TEST INFRASTRUCTURE: the tracing kernel's per-lane
+// logic (grt_device.hpp: lane_iterate, the separable cell search, ...) and
+// the library's host-side grid preparation (grt_host_prep.hpp), compiled for
+// the CPU so their results can be compared with the oracle without a GPU.
+// Photons run one after another to termination or max_steps calls, which is
+// the reference loop's outcome for termination_percent = 1.
+#include <cstring>
+#include <vector>
+
+#include "grt.h"
+#include "grt_device.hpp"
+#include "grt_host_prep.hpp"
+
+using namespace grt;
+
+namespace {
+
+struct Prepared {
+    HostGrid hg;
+    std::vector<double> coords;
+    GridView view;
+};
+
+void prepare(const grt_params* p, int nr, int nth, int nph, const double* r, const double* th,
+             const double* ph, const double* rho, const double* tg, const double* vel,
+             const double* mag, int force_general, Prepared& out)
+{
+    const bool avail = nr > 0 && r != nullptr;
+    if (avail) {
+        prepare_grid(*p, nr, nth, nph, r, th, ph, rho, tg, vel, mag, out.hg);
+        if (force_general) out.hg.separable = false;
+        if (!out.hg.separable) {
+            const size_t G = (size_t)nr * nth * nph;
+            out.coords.resize(3 * G);
+            std::memcpy(out.coords.data(), r, G * 8);
+            std::memcpy(out.coords.data() + G, th, G * 8);
+            std::memcpy(out.coords.data() + 2 * G, ph, G * 8);
+        }
+    }
+    out.view = grid_view(nr, nth, nph, avail, avail && out.hg.separable, out.hg.tables.data(),
+                         out.coords.data(), out.hg.cells.data());
+}
+
+}  // namespace
+
+extern "C" {
+
+int harness_locate(const grt_params* p, int nr, int nth, int nph, const double* r,
+                   const double* th, const double* ph, const double* rho, const double* tg,
+                   const double* vel, const double* mag, int force_general, int64_t nq,
+                   const double* radius, const double* polar, const double* az, int32_t* out)
+{
+    Prepared pr;
+    prepare(p, nr, nth, nph, r, th, ph, rho, tg, vel, mag, force_general, pr);
+    for (int64_t q = 0; q < nq; ++q) {
+        int i = -1, j = -1, k = -1;
+        const bool ok = pr.view.separable
+                            ? locate_separable(pr.view, radius[q], polar[q], az[q], i, j, k)
+                            : locate_general(pr.view, radius[q], polar[q], az[q], i, j, k);
+        out[4 * q + 0] = ok ? 1 : 0;
+        out[4 * q + 1] = i;
+        out[4 * q + 2] = j;
+        out[4 * q + 3] = k;
+    }
+    return pr.view.separable ? 1 : 0;
+}
+
+int harness_trace(const grt_params* p, int nr, int nth, int nph, const double* r,
+                  const double* th, const double* ph, const double* rho, const double* tg,
+                  const double* vel, const double* mag, int force_general, grt_photons* P,
+                  uint64_t n, int max_steps, uint64_t* stats)
+{
+    Prepared pr;
+    prepare(p, nr, nth, nph, r, th, ph, rho, tg, vel, mag, force_general, pr);
+    StepParams sp = make_step_params(*p);
+    const bool fluid = pr.view.available;
+    sp.absorb = sp.absorb && fluid;
+    sp.emit = sp.emit && fluid;
+    sp.scatter = sp.scatter && fluid;
+    LaneCounters cnt{0, 0, 0, 0, 0};
+    for (int s = 0; s < 5; ++s) stats[s] = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (P->terminate[i]) continue;
+        Lane L;
+        Photon& f = L.ph;
+        f.s[0] = P->x0[i]; f.s[1] = P->x1[i]; f.s[2] = P->x2[i]; f.s[3] = P->x3[i];
+        f.s[4] = P->k0[i]; f.s[5] = P->k1[i]; f.s[6] = P->k2[i]; f.s[7] = P->k3[i];
+        f.stokes[0] = P->I[i]; f.stokes[1] = P->Q[i]; f.stokes[2] = P->U[i]; f.stokes[3] = P->V[i];
+        f.freq = P->frequency[i];
+        f.eem = P->emission_frame_energy[i];
+        f.dl = P->dlambda[i];
+        f.phase = P->phase[i];
+        f.term = 0;
+        lane_reset(L);
+        PhotonRng rng{P->id[i], sp.seed, sp.stream << 1, 0u};
+        int calls = 0;
+        double au = 0.0, av = 0.0;
+        while (calls < max_steps && !f.term) {
+            au = av = 0.0;
+            if (sp.integrator == GRT_RK4) {
+                if (lane_iterate<GRT_RK4>(L, pr.view, sp, rng, cnt, au, av)) { ++calls; ++cnt.calls; }
+            } else {
+                if (lane_iterate<GRT_RK45>(L, pr.view, sp, rng, cnt, au, av)) { ++calls; ++cnt.calls; }
+            }
+        }
+        P->x0[i] = f.s[0]; P->x1[i] = f.s[1]; P->x2[i] = f.s[2]; P->x3[i] = f.s[3];
+        P->k0[i] = f.s[4]; P->k1[i] = f.s[5]; P->k2[i] = f.s[6]; P->k3[i] = f.s[7];
+        P->I[i] = f.stokes[0]; P->Q[i] = f.stokes[1]; P->U[i] = f.stokes[2]; P->V[i] = f.stokes[3];
+        P->frequency[i] = f.freq;
+        P->emission_frame_energy[i] = f.eem;
+        P->dlambda[i] = f.dl;
+        P->phase[i] = f.phase;
+        P->terminate[i] = f.term;
+        if (f.phase == kArrived) {
+            P->theta_disp[i] = au;
+            P->phi_disp[i] = av;
+        }
+    }
+    stats[0] = cnt.calls;
+    stats[1] = cnt.accepted;
+    stats[2] = cnt.rejected;
+    stats[3] = cnt.failed;
+    stats[4] = cnt.samples;
+    return 0;
+}
+
+}  // extern "C"

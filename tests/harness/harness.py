@@ -1,0 +1,69 @@
+"""ctypes wrapper of libgrt_host_harness.so (TEST INFRASTRUCTURE): the
+tracing kernel's per-lane logic compiled for the CPU."""
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(REPO, "gr-ray-tracer_amd"))
+import grt_amd  # noqa: E402  (only for the Params / Photons layouts)
+
+LIB = os.path.join(HERE, "libgrt_host_harness.so")
+_lib = None
+
+
+def build(force=False):
+    if force or not os.path.exists(LIB):
+        subprocess.run(["sh", os.path.join(HERE, "build.sh")], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(LIB)
+        vp = C.c_void_p
+        L.harness_trace.restype = C.c_int
+        L.harness_trace.argtypes = [C.POINTER(grt_amd.Params), C.c_int, C.c_int, C.c_int] + \
+            [vp] * 7 + [C.c_int, C.POINTER(grt_amd.Photons), C.c_uint64, C.c_int,
+                        C.POINTER(C.c_uint64)]
+        L.harness_locate.restype = C.c_int
+        L.harness_locate.argtypes = [C.POINTER(grt_amd.Params), C.c_int, C.c_int, C.c_int] + \
+            [vp] * 7 + [C.c_int, C.c_int64, vp, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _grid_args(fields):
+    if fields is None:
+        return [0, 0, 0] + [None] * 7, []
+    arr = [np.ascontiguousarray(fields[k], dtype=np.float64)
+           for k in ("r", "theta", "phi", "rho", "Tgas", "vel", "mag")]
+    nr, nth, nph = arr[0].shape
+    return [nr, nth, nph] + [a.ctypes.data for a in arr], arr
+
+
+def trace(params, fields, photons, max_steps, force_general=False):
+    args, keep = _grid_args(fields)
+    st = (C.c_uint64 * 5)()
+    s = grt_amd.photon_struct(photons)
+    lib().harness_trace(C.byref(params), *args, int(force_general), C.byref(s),
+                        len(photons["x0"]), max_steps, st)
+    return dict(functor_calls=st[0], accepted_steps=st[1], rejected_attempts=st[2],
+                failed_steps=st[3], fluid_samples=st[4])
+
+
+def locate(params, fields, radius, polar, az, force_general=False):
+    args, keep = _grid_args(fields)
+    radius = np.ascontiguousarray(radius, dtype=np.float64)
+    polar = np.ascontiguousarray(polar, dtype=np.float64)
+    az = np.ascontiguousarray(az, dtype=np.float64)
+    out = np.zeros((len(radius), 4), dtype=np.int32)
+    sep = lib().harness_locate(C.byref(params), *args, int(force_general), len(radius),
+                               radius.ctypes.data, polar.ctypes.data, az.ctypes.data,
+                               out.ctypes.data)
+    return out, bool(sep)
