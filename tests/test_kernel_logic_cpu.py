@@ -1,0 +1,161 @@
+"""The tracing kernel's per-lane logic (grt_device.hpp, compiled for the CPU
+by tests/harness) against the oracle on the first reference configuration
+and on variants exercising every branch of the functor: RK4, the general
+(non-separable) cell search, scattering (same Philox stream, so per-photon
+exact), disk-mode packets, horizon capture, 12-failure termination.
+
+Compiled for the CPU without FMA the kernel logic reproduces the oracle to
+the last bit for positions, momenta, Stokes and step sizes; the fluid tetrad
+uses the factored metric, so the emission-frame energy (and the redshifted
+frequency) agrees to ~1e-15 relative.
+"""
+import numpy as np
+import pytest
+
+from conftest import c1_fields, c1_params, pixel_screen
+
+STATE = ["x0", "x1", "x2", "x3", "k0", "k1", "k2", "k3", "I", "Q", "U", "V", "dlambda",
+         "theta_disp", "phi_disp"]
+
+
+UNIT_KEYS = ("length_cm_per_code", "time_s_per_code", "density_g_cm3_per_code",
+             "temperature_k_per_code", "four_velocity_cm_s_per_code", "magnetic_gauss_per_code")
+
+
+def split_units(kw):
+    """(oracle params kwargs, oracle grid units) from a grt_params kwargs dict."""
+    return ({k: v for k, v in kw.items() if k not in UNIT_KEYS},
+            {k: v for k, v in kw.items() if k in UNIT_KEYS})
+
+
+def run(oracle, fields, kw, ph0, max_steps=10000, force_general=False):
+    import grt_amd
+    import harness
+    A = {k: v.copy() for k, v in ph0.items()}
+    B = {k: v.copy() for k, v in ph0.items()}
+    okw, units = split_units(kw)
+    G = oracle.GridHolder(fields, kw["a_bh"], kw["m_bh"], units) if fields is not None else None
+    st_o = oracle.trace(oracle.make_params(**okw), G, A, max_steps, 1.0, 0, 4)
+    st_h = harness.trace(grt_amd.default_params(**kw), fields, B, max_steps, force_general)
+    return A, B, st_o, st_h
+
+
+def assert_same(A, B, st_o, st_h, freq_tol=1e-14):
+    for k in STATE:
+        assert np.array_equal(A[k], B[k], equal_nan=True), k
+    assert np.array_equal(A["phase"], B["phase"])
+    assert np.array_equal(A["terminate"], B["terminate"])
+    f = np.abs(A["frequency"] - B["frequency"]) / np.maximum(np.abs(A["frequency"]), 1e-300)
+    assert f.max() <= freq_tol
+    for k in ("functor_calls", "accepted_steps", "rejected_attempts", "failed_steps",
+              "fluid_samples"):
+        assert st_o[k] == st_h[k], k
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return c1_fields()
+
+
+def test_c1_image_rk45(oracle, c1):
+    kw = c1_params()
+    su, sv = pixel_screen(48)
+    ph0 = oracle.init_image(oracle.make_params(**kw), su, sv, 0.1)
+    A, B, st_o, st_h = run(oracle, c1, kw, ph0)
+    assert st_o["accepted_steps"] > 20 * len(su)
+    assert_same(A, B, st_o, st_h)
+    assert np.all(A["phase"] == 3)
+
+
+def test_c1_image_general_search(oracle, c1):
+    kw = c1_params()
+    su, sv = pixel_screen(12)
+    ph0 = oracle.init_image(oracle.make_params(**kw), su, sv, 0.1)
+    A, B, st_o, st_h = run(oracle, c1, kw, ph0, force_general=True)
+    assert_same(A, B, st_o, st_h)
+
+
+def test_torus_image_rk4_and_horizon(oracle):
+    """RK4 with a fixed step through a floor-free torus: rays are captured by
+    the horizon, escape, or emit; every outcome must match."""
+    from grt_amd.fm_torus import fishbone_moncrief
+    fields = fishbone_moncrief(nr=64, ntheta=32, nphi=32, r_min=1.5, r_max=100.0, spin=0.9,
+                               r_in=6.0, r_pressure_max=9.0, rho_floor=0.0)
+    kw = c1_params(theta=np.pi / 4, integrator=0, camera_width=30.0, camera_height=30.0)
+    su, sv = pixel_screen(24, width=30.0, height=30.0)
+    ph0 = oracle.init_image(oracle.make_params(**kw), su, sv, 0.002)
+    A, B, st_o, st_h = run(oracle, fields, kw, ph0, max_steps=3000)
+    assert_same(A, B, st_o, st_h)
+    counts = np.bincount(A["phase"], minlength=5)
+    assert counts[4] > 0 and counts[3] > 0
+
+
+def test_scattering_per_photon_exact(oracle, c1):
+    kw = c1_params(scattering_enabled=1, albedo=0.7, seed=99,
+                   length_cm_per_code=1e24)   # optical depth of order one per step
+    su, sv = pixel_screen(24)
+    ph0 = oracle.init_image(oracle.make_params(**split_units(kw)[0]), su, sv, 0.1)
+    A, B, st_o, st_h = run(oracle, c1, kw, ph0, max_steps=2000)
+    assert_same(A, B, st_o, st_h)
+
+
+def disk_packets(oracle, fields, kw):
+    okw, units = split_units(kw)
+    G = oracle.GridHolder(fields, kw["a_bh"], 1.0, units)
+    return oracle.init_disk(oracle.make_params(**okw), G, 0.1)
+
+
+def test_disk_mode_packets(oracle):
+    fields = c1_fields(nr=16, nth=8, nph=8)
+    kw = c1_params(mode=1, superphoton_count_normalization=3.0, generator=1, nu_min_hz=1e9,
+                   nu_max_hz=1e12)
+    ph0 = disk_packets(oracle, fields, kw)
+    assert len(ph0["x0"]) > 100
+    A, B, st_o, st_h = run(oracle, fields, kw, ph0, max_steps=3000)
+    assert_same(A, B, st_o, st_h)
+
+
+def test_disk_mode_multiple_scattering(oracle):
+    """With repeated scattering, last-bit differences of the fluid tetrad (the
+    kernel's factored metric) are amplified along each packet's path, so the
+    comparison is statistical: same termination pattern and the same
+    distribution of escape directions and intensities."""
+    fields = c1_fields(nr=16, nth=8, nph=8)
+    kw = c1_params(mode=1, superphoton_count_normalization=6.0, scattering_enabled=1,
+                   albedo=0.9, length_cm_per_code=1e23, generator=2)
+    ph0 = disk_packets(oracle, fields, kw)
+    A, B, st_o, st_h = run(oracle, fields, kw, ph0, max_steps=3000)
+    n = len(A["x0"])
+    assert n > 300
+    assert np.mean(A["phase"] == B["phase"]) > 0.98
+    from scipy import stats
+    for f in (lambda P: np.arctan2(P["x2"], P["x1"]),
+              lambda P: P["x3"] / np.sqrt(P["x1"] ** 2 + P["x2"] ** 2 + P["x3"] ** 2),
+              lambda P: P["I"]):
+        assert stats.ks_2samp(f(A), f(B)).pvalue > 0.01
+    assert abs(st_o["accepted_steps"] - st_h["accepted_steps"]) < 0.02 * st_o["accepted_steps"]
+
+
+def test_twelve_failures_terminate(oracle, c1):
+    kw = c1_params(atol=1e-300, rtol=1e-300)
+    su, sv = pixel_screen(4)
+    ph0 = oracle.init_image(oracle.make_params(**kw), su, sv, 0.1)
+    A, B, st_o, st_h = run(oracle, c1, kw, ph0, max_steps=50)
+    assert_same(A, B, st_o, st_h)
+    # steps whose embedded error is exactly zero are still accepted (err == 0
+    # -> x max_scale); any other error exceeds a 1e-300 tolerance 12 times.
+    assert st_o["failed_steps"] >= 1
+    failed = (A["terminate"] == 1) & (A["phase"] == 1)     # phase unchanged on failure
+    assert failed.sum() == st_o["failed_steps"]
+
+
+def test_vacuum_nan_and_inside_horizon(oracle):
+    kw = c1_params(absorption_enabled=0, emission_enabled=0)
+    su, sv = pixel_screen(4)
+    ph0 = oracle.init_image(oracle.make_params(**kw), su, sv, 0.1)
+    ph0["x1"][0] = np.nan
+    ph0["x1"][1], ph0["x2"][1], ph0["x3"][1] = 0.5, 0.2, 0.1      # inside r_min
+    ph0["I"][1] = 5.0
+    A, B, st_o, st_h = run(oracle, None, kw, ph0, max_steps=400)
+    assert_same(A, B, st_o, st_h)
+    assert A["phase"][1] == 4 and A["I"][1] == 0.0

@@ -1,0 +1,211 @@
+"""Pins the oracle's RK4 / RKF45 integrators (rk_integrators.hpp,
+integrate_cart_ks.hpp:143-227) with the reference's own tests
+(test_rk_integrators.cpp, test_convergence.cpp) and with exact rational
+consequences of the Fehlberg 4(5) tableau computed here from its published
+coefficients (Fehlberg 1969): the stability polynomial of the 5th-order
+solution and the local error constant of the embedded pair.
+"""
+import ctypes as C
+from fractions import Fraction as Fr
+
+import numpy as np
+
+# Fehlberg's RKF45 coefficients (Fehlberg, NASA TR R-315, 1969, Table III)
+A = [[], [Fr(1, 4)], [Fr(3, 32), Fr(9, 32)], [Fr(1932, 2197), Fr(-7200, 2197), Fr(7296, 2197)],
+     [Fr(439, 216), Fr(-8), Fr(3680, 513), Fr(-845, 4104)],
+     [Fr(-8, 27), Fr(2), Fr(-3544, 2565), Fr(1859, 4104), Fr(-11, 40)]]
+B4 = [Fr(25, 216), 0, Fr(1408, 2565), Fr(2197, 4104), Fr(-1, 5), 0]
+B5 = [Fr(16, 135), 0, Fr(6656, 12825), Fr(28561, 56430), Fr(-9, 50), Fr(2, 55)]
+
+
+def _cb(fn):
+    def f(inp, out, _user):
+        fn(inp, out)
+    return f
+
+
+def rk45(oracle, n, state, dt, fn, atol=1e-8, rtol=1e-8):
+    L = oracle.lib()
+    cb = oracle._RHS_FN(_cb(fn))
+    s = np.ascontiguousarray(state, dtype=np.float64)
+    k1 = np.zeros(n)
+    fn(s.ctypes.data_as(oracle._D), k1.ctypes.data_as(oracle._D))
+    h = np.array([dt])
+    acc = L.orc_rk45_generic(n, oracle.dptr(s), oracle.dptr(h), oracle.dptr(k1), cb, None,
+                             atol, rtol, 0.01, 10.0, 0.9)
+    return bool(acc), s, h[0]
+
+
+def rk4(oracle, n, state, dt, fn):
+    cb = oracle._RHS_FN(_cb(fn))
+    s = np.ascontiguousarray(state, dtype=np.float64)
+    oracle.lib().orc_rk4_generic(n, oracle.dptr(s), dt, cb, None)
+    return s
+
+
+def test_reference_linear_system(oracle):
+    # test_rk_integrators.cpp: x' = v, v' = 0 from (1, -2) with dt = 0.5
+    def f(i, o):
+        o[0] = i[1]
+        o[1] = 0.0
+    s = rk4(oracle, 2, [1.0, -2.0], 0.5, f)
+    assert abs(s[0]) < 1e-14 and abs(s[1] + 2.0) < 1e-14
+    acc, s, h = rk45(oracle, 2, [1.0, -2.0], 0.5, f)
+    assert acc and abs(s[0]) < 1e-14 and abs(s[1] + 2.0) < 1e-14
+
+
+def test_fifth_order_stability_polynomial(oracle):
+    """y' = y: the 5th-order solution is R(h) = sum_k b5^T A^(k-1) 1 h^k / ..."""
+    # R(z) = 1 + z b^T (I - zA)^-1 1, expanded exactly for the 6-stage tableau
+    import itertools
+    Am = [[A[i][j] if j < i else Fr(0) for j in range(6)] for i in range(6)]
+
+    def matvec(M, v):
+        return [sum(M[i][j] * v[j] for j in range(6)) for i in range(6)]
+
+    h = Fr(1, 2)
+    v = [Fr(1)] * 6
+    coeffs = []
+    for _ in range(6):
+        coeffs.append(sum(B5[i] * v[i] for i in range(6)))
+        v = matvec(Am, v)
+    R = 1 + sum(c * h ** (k + 1) for k, c in enumerate(coeffs))
+    assert coeffs[:5] == [Fr(1), Fr(1, 2), Fr(1, 6), Fr(1, 24), Fr(1, 120)]
+
+    def f(i, o):
+        o[0] = i[0]
+    acc, s, _ = rk45(oracle, 1, [1.0], 0.5, f, atol=1.0, rtol=1.0)
+    assert acc
+    assert abs(s[0] - float(R)) < 4e-16
+    del itertools
+
+
+def test_embedded_error_constant(oracle):
+    """y' = t^4 (t' = 1): x5 is exact, x5 - x4 = h^5 sum_j (b5_j - b4_j) c_j^4,
+    so the proposed step is h * 0.9 * err^(-1/4) with that err."""
+    c = [sum(row, Fr(0)) for row in A]
+    e4 = sum((B5[j] - B4[j]) * c[j] ** 4 for j in range(6))
+    h = 0.5
+
+    def f(i, o):
+        o[0] = 1.0
+        o[1] = i[0] ** 4
+    atol = 1e-3
+    acc, s, hn = rk45(oracle, 2, [0.0, 0.0], h, f, atol=atol, rtol=0.0)
+    assert abs(s[1] - h ** 5 / 5) < 1e-16                    # exact 5th-order solution
+    err = abs(float(e4) * h ** 5) / atol
+    expect = h * min(10.0, max(0.01, 0.9 * err ** -0.25))
+    assert abs(hn - expect) < 1e-12 * expect
+    assert acc == (err <= 1.0)
+
+
+def test_accept_reject_rules(oracle):
+    def lin(i, o):
+        o[0] = 1.0
+    # err == 0 -> accepted, step x max_scale
+    acc, s, hn = rk45(oracle, 1, [0.0], 0.25, lin)
+    assert acc and hn == 2.5 and s[0] == 0.25
+
+    def blow(i, o):
+        o[0] = np.inf
+    acc, s, hn = rk45(oracle, 1, [1.0], 0.5, blow)
+    assert not acc and hn == 0.5 * 0.01 and s[0] == 1.0
+
+    def stiff(i, o):
+        o[0] = -1e6 * i[0]
+    acc, s, hn = rk45(oracle, 1, [1.0], 1.0, stiff)
+    assert not acc and s[0] == 1.0 and abs(hn - 0.01) < 1e-15   # scale clamped to min
+
+
+def test_reference_convergence(oracle):
+    # test_convergence.cpp:62-96 (a = M = 1)
+    ok, p = oracle.null_momentum(np.array([0.0, 6.0, 2.0, 1.0]), [-0.4, 0.8, 0.2], 1.0, 1.0)
+    assert ok
+    s0 = np.array([0.0, 6.0, 2.0, 1.0, *p])
+
+    def run(steps):
+        s = s0.copy()
+        dt = 4.0 / steps
+        for _ in range(steps):
+            oracle.lib().orc_rk4_step_ks(oracle.dptr(s), dt, 1.0, 1.0)
+        return s
+    ref = run(8192)
+    e8, e16, e32 = (np.abs(run(n) - ref).max() for n in (8, 16, 32))
+    assert e8 / e16 > 10 and e16 / e32 > 10
+
+    def osc(tol):
+        def f(i, o):
+            o[0] = i[1]
+            o[1] = -i[0]
+        s = [1.0, 0.0]
+        t, dt, attempts = 0.0, 0.2, 0
+        while t < 2.0 and attempts < 100000:
+            attempts += 1
+            h = min(dt, 2.0 - t)
+            acc, s2, hn = rk45(oracle, 2, s, h, f, atol=tol, rtol=tol)
+            dt = hn
+            if acc:
+                t += h
+                s = list(s2)
+        return max(abs(s[0] - np.cos(2.0)), abs(s[1] + np.sin(2.0)))
+    assert osc(1e-7) < osc(1e-5) and osc(1e-9) < osc(1e-7)
+
+
+def carter_null(s, a, r):
+    """Carter constant of a photon, Q = p_th^2 + cos^2 th (L_z^2 / sin^2 th - a^2 E^2)
+    (Carter 1968), with p_th = dx^i/dth p_i for the oblate Kerr-Schild map
+    x = (r cos ph - a sin ph) sin th, y = (r sin ph + a cos ph) sin th, z = r cos th.
+    (The reference's observable at geodesic_observables.hpp:30-31 adds a^2 E^2
+    instead, which is not conserved for a != 0.)"""
+    x, y, z = s[1:4]
+    px, py, pz = s[5:8]
+    ct = z / r
+    st = np.sqrt(1 - ct * ct)
+    E = -s[4]
+    Lz = x * py - y * px
+    pth = (x * px + y * py) * ct / st - r * st * pz
+    return pth * pth + ct * ct * (Lz * Lz / (st * st) - a * a * E * E)
+
+
+def test_conservation_along_geodesic(oracle):
+    """p_t, L_z (Killing vectors d_t, x d_y - y d_x), the Carter constant and
+    the null norm stay constant along an RK45 trajectory."""
+    a, m = 0.9, 1.0
+    p = oracle.make_params(a_bh=a, m_bh=m, atol=1e-12, rtol=1e-12)
+    ok, k = oracle.null_momentum(np.array([0.0, 30.0, 5.0, 4.0]), [-1.0, 0.15, -0.05], a, m)
+    s = np.array([0.0, 30.0, 5.0, 4.0, *k])
+    s0 = s.copy()
+    o0 = oracle.observables(s, a, m)
+    h = np.array([0.5])
+    L = oracle.lib()
+    rmin = 1e9
+    for _ in range(400):
+        k1 = oracle.rhs(s[:4], s[4:], a, m)
+        L.orc_rk45_step_ks(C.byref(p), oracle.dptr(s), oracle.dptr(h), oracle.dptr(k1))
+        rmin = min(rmin, L.orc_compute_r(s[1], s[2], s[3], a))
+    o1 = oracle.observables(s, a, m)
+    assert rmin < 10.0                       # it actually passed through the strong field
+    assert o1["energy"] == o0["energy"]
+    assert abs(o1["angular_momentum_z"] - o0["angular_momentum_z"]) < 1e-9
+    q0 = carter_null(s0, a, L.orc_compute_r(s0[1], s0[2], s0[3], a))
+    q1 = carter_null(s, a, L.orc_compute_r(s[1], s[2], s[3], a))
+    assert abs(q1 - q0) < 1e-8 * max(1.0, abs(q0))
+    assert abs(o1["norm"]) < 1e-9
+
+
+def test_photon_sphere_schwarzschild(oracle):
+    """A tangential photon at r = 3M stays near r = 3M (unstable circular orbit)."""
+    m = 1.0
+    X = np.array([0.0, 3.0, 0.0, 0.0])
+    ok, k = oracle.null_momentum(X, [0.0, 1.0, 0.0], 0.0, m)
+    s = np.array([*X, *k])
+    p = oracle.make_params(a_bh=0.0, m_bh=m, atol=1e-13, rtol=1e-13)
+    h = np.array([0.01])
+    L = oracle.lib()
+    phi = 0.0
+    for _ in range(300):
+        k1 = oracle.rhs(s[:4], s[4:], 0.0, m)
+        L.orc_rk45_step_ks(C.byref(p), oracle.dptr(s), oracle.dptr(h), oracle.dptr(k1))
+    r = np.linalg.norm(s[1:4])
+    phi = np.arctan2(s[2], s[1])
+    assert abs(r - 3.0) < 1e-6 and abs(phi) > 0.5

@@ -1,0 +1,185 @@
+"""Pins the oracle's nearest-cell search (scattering.hpp:41-180), fluid
+tetrad (fluid_frame.hpp:52-80) and radiative-transfer update
+(radiative_transfer.hpp:54-96): the reference's own known answers
+(test_npy_loader.cpp:74-97, test_radiative_transfer.cpp:14-79), brute-force
+nearest search on small grids, and g(e_a, e_b) = eta_ab for the tetrad.
+It also checks the tracing kernel's separable fast path (binary search +
+bracketing candidates, compiled for the CPU) against the oracle's scans.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+
+def npy_loader_grid():
+    # test_npy_loader.cpp:12-40: a deliberately nonuniform 3x2x4 grid
+    shape = (3, 2, 4)
+    r = np.zeros(shape)
+    th = np.zeros(shape)
+    ph = np.zeros(shape)
+    for i in range(3):
+        for j in range(2):
+            for k in range(4):
+                r[i, j, k] = 1.0 + i * i + 0.01 * j + 0.001 * k
+                th[i, j, k] = 0.1 + 0.2 * j * j + 0.005 * i
+                ph[i, j, k] = 0.05 + 0.3 * k * k + 0.002 * j
+    rho = np.arange(1, 25, dtype=np.float64).reshape(shape)
+    tg = 100.0 + np.arange(24, dtype=np.float64).reshape(shape)
+    vel = 0.01 * np.arange(1, 97, dtype=np.float64).reshape(shape + (4,))
+    mag = 0.02 * np.arange(1, 97, dtype=np.float64).reshape(shape + (4,))
+    return dict(r=r, theta=th, phi=ph, rho=rho, Tgas=tg, vel=vel, mag=mag)
+
+
+def locate(oracle, G, radius, polar, az):
+    ijk = (C.c_int32 * 3)(-1, -1, -1)
+    ok = oracle.lib().orc_locate_cell(C.byref(G.grid), radius, polar, az, ijk)
+    return bool(ok), tuple(ijk)
+
+
+def test_reference_nonuniform_lookup(oracle):
+    f = npy_loader_grid()
+    G = oracle.GridHolder(f, 1.0, 1.0)
+    ok, ijk = locate(oracle, G, f["r"][2, 1, 3], f["theta"][2, 1, 3], f["phi"][2, 1, 3])
+    assert ok and ijk == (2, 1, 3)
+
+
+def brute_nearest(axis, v, dist):
+    d = np.array([dist(v, c) for c in axis])
+    return int(np.argmin(d))        # first minimum, like the strict '<' scans
+
+
+def periodic(a, b):
+    d = np.fmod(abs(a - b), 2 * np.pi)
+    return min(d, 2 * np.pi - d)
+
+
+def separable_grid(rng, nr, nth, nph):
+    ra = np.sort(rng.uniform(1.5, 100.0, nr))
+    ta = np.sort(rng.uniform(0.001, np.pi - 0.001, nth))
+    pa = np.sort(rng.uniform(0.0, 2 * np.pi, nph))
+    R, T, P = np.meshgrid(ra, ta, pa, indexing="ij")
+    shape = R.shape
+    return dict(r=R, theta=T, phi=P, rho=np.ones(shape), Tgas=np.ones(shape),
+                vel=np.zeros(shape + (4,)), mag=np.zeros(shape + (4,))), ra, ta, pa
+
+
+def test_search_equals_brute_force_on_separable_grids(oracle):
+    rng = np.random.default_rng(11)
+    for trial in range(6):
+        f, ra, ta, pa = separable_grid(rng, 7 + trial, 5 + trial, 9)
+        G = oracle.GridHolder(f, 0.5, 1.0)
+        for _ in range(300):
+            radius = rng.uniform(1.0, 120.0)
+            polar = rng.uniform(0.0, np.pi)
+            az = rng.uniform(0.0, 2 * np.pi)
+            ok, (i, j, k) = locate(oracle, G, radius, polar, az)
+            assert i == brute_nearest(ra, radius, lambda v, c: abs(np.log(v / c)))
+            assert j == brute_nearest(ta, polar, lambda v, c: abs(v - c))
+            assert k == brute_nearest(pa, az, periodic)
+
+
+def test_kernel_fast_search_matches_oracle(oracle):
+    """The separable fast path of the HIP kernel (compiled for the CPU) returns
+    the same (found, i, j, k) as the oracle's linear scans, including points
+    outside the grid, on axis ends, exactly on cell centres and on midpoints."""
+    import grt_amd
+    import harness
+    rng = np.random.default_rng(3)
+    p = grt_amd.default_params(a_bh=0.5)
+    for nr, nth, nph in ((64, 32, 32), (5, 3, 7), (1, 4, 6), (3, 1, 1), (17, 9, 2)):
+        f, ra, ta, pa = separable_grid(rng, nr, nth, nph)
+        G = oracle.GridHolder(f, 0.5, 1.0)
+        q = 4000
+        radius = np.exp(rng.uniform(np.log(0.5), np.log(300.0), q))
+        polar = rng.uniform(0.0, np.pi, q)
+        az = rng.uniform(0.0, 2 * np.pi, q)
+        # exact centres, midpoints and axis ends
+        extra_r = np.concatenate([ra, np.sqrt(ra[1:] * ra[:-1]), [ra[0], ra[-1]]])
+        extra_t = np.concatenate([ta, 0.5 * (ta[1:] + ta[:-1]), [0.0, np.pi]])
+        extra_p = np.concatenate([pa, 0.5 * (pa[1:] + pa[:-1]), [0.0, 2 * np.pi - 1e-12]])
+        m = max(len(extra_r), len(extra_t), len(extra_p))
+        radius = np.concatenate([radius, np.resize(extra_r, m)])
+        polar = np.concatenate([polar, np.resize(extra_t, m)])
+        az = np.concatenate([az, np.resize(extra_p, m)])
+        out, sep = harness.locate(p, f, radius, polar, az)
+        assert sep
+        for n in range(len(radius)):
+            ok, ijk = locate(oracle, G, radius[n], polar[n], az[n])
+            assert (bool(out[n, 0]), tuple(out[n, 1:])) == (ok, ijk), (nr, nth, nph, n)
+
+
+def test_kernel_general_search_matches_oracle(oracle):
+    import grt_amd
+    import harness
+    f = npy_loader_grid()
+    G = oracle.GridHolder(f, 1.0, 1.0)
+    rng = np.random.default_rng(5)
+    radius = rng.uniform(0.5, 12.0, 2000)
+    polar = rng.uniform(0.0, 1.2, 2000)
+    az = rng.uniform(0.0, 2 * np.pi, 2000)
+    out, sep = harness.locate(grt_amd.default_params(), f, radius, polar, az)
+    assert not sep
+    for n in range(len(radius)):
+        ok, ijk = locate(oracle, G, radius[n], polar[n], az[n])
+        assert bool(out[n, 0]) == ok and tuple(out[n, 1:]) == ijk
+
+
+def test_tetrad_orthonormal(oracle):
+    rng = np.random.default_rng(2)
+    eta = np.diag([-1.0, 1.0, 1.0, 1.0])
+    L = oracle.lib()
+    for _ in range(40):
+        a = rng.uniform(0, 0.99)
+        X = np.array([0.0, *rng.uniform(-20, 20, 3)])
+        if L.orc_compute_r(X[1], X[2], X[3], a) < 3.0:
+            continue
+        g = oracle.metric(X, a, 1.0)
+        # a timelike vector: static observer boosted a little
+        u = np.array([1.0, *rng.uniform(-0.1, 0.1, 3)])
+        if u @ g @ u >= 0:
+            continue
+        e = np.zeros(16)
+        assert L.orc_build_fluid_tetrad(oracle.dptr(X), oracle.dptr(u), a, 1.0, oracle.dptr(e))
+        E = e.reshape(4, 4)
+        assert np.allclose(E @ g @ E.T, eta, atol=1e-12)
+        assert np.allclose(np.cross(E[0, 1:], u[1:]), 0, atol=1e-12) and E[0, 0] > 0
+
+
+def _rt(oracle, stokes, rho, T, dl, absorb, emit, ka, ke):
+    f = oracle.Fluid()
+    f.density_g_cm3 = rho
+    f.temperature_k = T
+    s = np.array(stokes, dtype=np.float64)
+    oracle.lib().orc_apply_radiative_transfer(oracle.dptr(s), C.byref(f), 1e14, dl, absorb, emit,
+                                              ka, ke)
+    return s
+
+
+def test_reference_radiative_transfer_cases(oracle):
+    # test_radiative_transfer.cpp:14-79
+    e1 = np.exp(-1.0)
+    s = _rt(oracle, [4.0, 2.0, 1.0, 0.5], 1.0, 2.0, 1.0, 1, 0, 1.0, 0.0)
+    assert np.allclose(s, np.array([4.0, 2.0, 1.0, 0.5]) * e1, atol=1e-12)
+    s = _rt(oracle, [4.0, 2.0, 1.0, 0.5], 1.0, 2.0, 1.0, 0, 1, 0.0, 1.0)
+    assert abs(s[0] - 6.0) < 1e-12 and list(s[1:]) == [2.0, 1.0, 0.5]
+    s = _rt(oracle, [4.0, 2.0, 1.0, 0.5], 1.0, 2.0, 1.0, 0, 0, 0.0, 0.0)
+    assert list(s) == [4.0, 2.0, 1.0, 0.5]
+    s = _rt(oracle, [4.0, 0.0, 0.0, 0.0], 1.0, 2.0, -1.0, 1, 0, 1.0, 0.0)
+    assert abs(s[0] - 4.0 * e1) < 1e-12
+    s = _rt(oracle, [4.0, 2.0, 1.0, 0.5], 0.0, 5.0, 1.0, 1, 1, 10.0, 10.0)
+    assert abs(s[0] - 4.0) < 1e-12 and list(s[1:]) == [2.0, 1.0, 0.5]
+
+
+def test_uniform_slab_solution(oracle):
+    """Many small steps through a uniform absorbing/emitting medium approach
+    the exact slab solution I = S + (I0 - S) e^{-tau} only to first order,
+    since the update emits without self-absorption; check the discrete
+    recursion I_{n+1} = I_n e^{-k dl} + e dl in closed form."""
+    k, e, dl, n = 0.3, 2.0, 0.01, 500
+    s = np.array([1.0, 0.0, 0.0, 0.0])
+    for _ in range(n):
+        s = _rt(oracle, s, 1.0, 1.0, dl, 1, 1, k, e)
+    q = np.exp(-k * dl)
+    exact = q ** n * 1.0 + e * dl * (1 - q ** n) / (1 - q)
+    assert abs(s[0] - exact) < 1e-12 * exact
