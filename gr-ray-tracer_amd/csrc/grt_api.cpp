@@ -45,6 +45,8 @@ struct grt_ctx {
     size_t rng_cap = 0;
     uint32_t* d_counters = nullptr;  // [0] queue, [1] n_out, [2] live count
     unsigned long long* d_stats = nullptr;
+    grt::TraceConsts* d_consts = nullptr;
+    grt::TraceConsts h_consts{};   // host source of the copy to d_consts
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     // device staging for the host-pointer entry points
@@ -301,6 +303,7 @@ int grt_create(const grt_params* p, grt_ctx** out)
     c->sp = grt::make_step_params(*p);
     if (hipMalloc(&c->d_counters, 8 * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&c->d_stats, grt::kStatCount * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&c->d_consts, sizeof(grt::TraceConsts)) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         grt_destroy(c);
         return fail(nullptr, GRT_E_NOMEM, "device allocation failed in grt_create");
@@ -313,7 +316,8 @@ int grt_destroy(grt_ctx* c)
 {
     if (!c) return GRT_OK;
     free_grid(c);
-    void* bufs[] = {c->d_lists, c->d_rng, c->d_counters, c->d_stats, c->d_stage, c->d_img};
+    void* bufs[] = {c->d_lists, c->d_rng, c->d_counters, c->d_stats, c->d_stage, c->d_img,
+                    c->d_consts};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -440,10 +444,14 @@ int grt_trace_device(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max
     GRT_HIP(c, hipMemcpyAsync(&live, c->d_counters + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     GRT_HIP(c, hipStreamSynchronize(stream));
 
+    c->h_consts.ph = to_ptrs(*dev, c->d_rng);
+    c->h_consts.P = P;
+    GRT_HIP(c, hipMemcpyAsync(c->d_consts, &c->h_consts, sizeof c->h_consts,
+                              hipMemcpyHostToDevice, stream));
     grt::TraceLaunch L{};
-    L.ph = to_ptrs(*dev, c->d_rng);
+    L.consts = c->d_consts;
+    L.integrator = P.integrator;
     L.grid = make_grid_view(c);
-    L.P = P;
     L.queue = c->d_counters + 0;
     L.n_out = c->d_counters + 1;
     L.stats = c->d_stats;

@@ -99,6 +99,10 @@ GRT_HD void ks_metric(const double X[4], double a, double m, double g[16])
 // Kerr-Schild form, kerr_schild_core.hpp:161-192 (with r, dr/dx^k of :36-46,
 // l, dl/dx^k of :59-77 and H, dH/dx^k of :86-95 expanded for k = x, y, z).
 // out[4] (dp_t/dlambda) is identically zero and is not written.
+#ifndef GRT_RHS_RECIPROCALS
+#define GRT_RHS_RECIPROCALS 0
+#endif
+
 GRT_HD void ks_rhs(const double s[8], double a, double m, double out[8])
 {
     const double x = s[1], y = s[2], z = s[3];
@@ -109,38 +113,53 @@ GRT_HD void ks_rhs(const double s[8], double a, double m, double out[8])
     const double az = a * z;
     const double S = sqrt(A * A + 4.0 * (az * az));
     const double r = sqrt(0.5 * (A + S));
-    const double f = 1.0 + A / S;
     const double tr = 2.0 * r;
-    const double drx = x * f / tr;
-    const double dry = y * f / tr;
-    const double drz = (z * f + 2.0 * a2 * z / S) / tr;
-
     const double B = r * r + a2;
-    const double l1 = (r * x + a * y) / B;
-    const double l2 = (r * y - a * x) / B;
-    const double l3 = z / r;
-    const double dBx = tr * drx, dBy = tr * dry, dBz = tr * drz;
-    // dl[k][mu]; the Kronecker terms of :73-75 reduce to +-r, +-a or 1.
-    const double dl_x1 = ((drx * x + r) - l1 * dBx) / B;
-    const double dl_x2 = ((drx * y - a) - l2 * dBx) / B;
-    const double dl_x3 = (-(l3 * drx)) / r;
-    const double dl_y1 = ((dry * x + a) - l1 * dBy) / B;
-    const double dl_y2 = ((dry * y + r) - l2 * dBy) / B;
-    const double dl_y3 = (-(l3 * dry)) / r;
-    const double dl_z1 = (drz * x - l1 * dBz) / B;
-    const double dl_z2 = (drz * y - l2 * dBz) / B;
-    const double dl_z3 = (1.0 - l3 * drz) / r;
-
     const double r2 = r * r;
     const double r3 = r2 * r;
     const double D = r3 * r + az * az;
-    const double H = m * r3 / D;
     const double D2 = D * D;
+#if GRT_RHS_RECIPROCALS
+    // Four reciprocals replace the 21 divisions (each quotient then differs
+    // from the reference's by at most an ulp or two).
+    const double iS = 1.0 / S, itr = 1.0 / tr, iB = 1.0 / B, iD = 1.0 / D;
+    const double ir = 2.0 * itr;
+    const double iD2 = iD * iD;
+    const double f = 1.0 + A * iS;
+    const double drx = x * f * itr;
+    const double dry = y * f * itr;
+    const double drz = (z * f + 2.0 * a2 * z * iS) * itr;
+#define GRT_DIV(n, d, inv) ((n) * (inv))
+#else
+    const double f = 1.0 + A / S;
+    const double drx = x * f / tr;
+    const double dry = y * f / tr;
+    const double drz = (z * f + 2.0 * a2 * z / S) / tr;
+#define GRT_DIV(n, d, inv) ((n) / (d))
+#endif
+
+    const double l1 = GRT_DIV(r * x + a * y, B, iB);
+    const double l2 = GRT_DIV(r * y - a * x, B, iB);
+    const double l3 = GRT_DIV(z, r, ir);
+    const double dBx = tr * drx, dBy = tr * dry, dBz = tr * drz;
+    // dl[k][mu]; the Kronecker terms of :73-75 reduce to +-r, +-a or 1.
+    const double dl_x1 = GRT_DIV((drx * x + r) - l1 * dBx, B, iB);
+    const double dl_x2 = GRT_DIV((drx * y - a) - l2 * dBx, B, iB);
+    const double dl_x3 = GRT_DIV(-(l3 * drx), r, ir);
+    const double dl_y1 = GRT_DIV((dry * x + a) - l1 * dBy, B, iB);
+    const double dl_y2 = GRT_DIV((dry * y + r) - l2 * dBy, B, iB);
+    const double dl_y3 = GRT_DIV(-(l3 * dry), r, ir);
+    const double dl_z1 = GRT_DIV(drz * x - l1 * dBz, B, iB);
+    const double dl_z2 = GRT_DIV(drz * y - l2 * dBz, B, iB);
+    const double dl_z3 = GRT_DIV(1.0 - l3 * drz, r, ir);
+
+    const double H = GRT_DIV(m * r3, D, iD);
     const double c3 = 3.0 * r2;
     const double c4 = 4.0 * r3;
-    const double dH_x = m * (c3 * drx * D - r3 * (c4 * drx)) / D2;
-    const double dH_y = m * (c3 * dry * D - r3 * (c4 * dry)) / D2;
-    const double dH_z = m * (c3 * drz * D - r3 * (c4 * drz + 2.0 * a2 * z)) / D2;
+    const double dH_x = GRT_DIV(m * (c3 * drx * D - r3 * (c4 * drx)), D2, iD2);
+    const double dH_y = GRT_DIV(m * (c3 * dry * D - r3 * (c4 * dry)), D2, iD2);
+    const double dH_z = GRT_DIV(m * (c3 * drz * D - r3 * (c4 * drz + 2.0 * a2 * z)), D2, iD2);
+#undef GRT_DIV
 
     // l^mu = (-1, l1, l2, l3)
     const double lp = ((-p0 + l1 * p1) + l2 * p2) + l3 * p3;
@@ -935,6 +954,10 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
     if (accepted) {
         ++n.accepted;
         L.in_call = false;
+        // k1 is recomputed when the next call opens; overwrite it so the
+        // register allocator need not keep it alive across the post-step.
+#pragma unroll
+        for (int q = 0; q < 8; ++q) L.k1[q] = 0.0;
         n.samples += post_step(ph, L.phase0, taken, G, P, rng, au, av) ? 1u : 0u;
         return true;
     }

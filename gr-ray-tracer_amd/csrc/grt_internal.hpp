@@ -37,10 +37,18 @@ enum StatSlot {
     kStatCount
 };
 
-struct TraceLaunch {
+// Launch constants kept in device memory: the kernel reads them through a
+// pointer at their use sites instead of pinning ~190 scalar registers for the
+// whole persistent loop.
+struct TraceConsts {
     PhotonPtrs ph;
-    GridView grid;
     StepParams P;
+};
+
+struct TraceLaunch {
+    const TraceConsts* consts;
+    GridView grid;
+    int integrator;
     const uint32_t* list_in;
     uint32_t n_in;
     uint32_t* queue;          // work counter over list_in, zeroed before launch

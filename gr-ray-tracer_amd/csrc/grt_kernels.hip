@@ -81,8 +81,12 @@ __device__ __forceinline__ void store_photon(const PhotonPtrs& p, uint32_t idx, 
     p.rng_draw[idx] = rng.draw;
 }
 
+#ifndef GRT_TRACE_WAVES_PER_EU
+#define GRT_TRACE_WAVES_PER_EU 2
+#endif
+
 template <int kIntegrator>
-__global__ __launch_bounds__(256, 2) void trace_fp64(TraceLaunch a)
+__global__ __launch_bounds__(256, GRT_TRACE_WAVES_PER_EU) void trace_fp64(TraceLaunch a)
 {
     extern __shared__ double lds[];
     GridView G = a.grid;
@@ -108,7 +112,8 @@ __global__ __launch_bounds__(256, 2) void trace_fp64(TraceLaunch a)
         G.ph_axis = lds + 2 * nr + 2 * nth;
         G.ph_thr = lds + 2 * nr + 2 * nth + G.nph;
     }
-    const StepParams& P = a.P;
+    const StepParams& P = a.consts->P;
+    const PhotonPtrs& PH = a.consts->ph;
 
     LaneCounters n{0u, 0u, 0u, 0u, 0u};
     uint32_t last_term = 0;
@@ -120,7 +125,7 @@ __global__ __launch_bounds__(256, 2) void trace_fp64(TraceLaunch a)
     uint32_t slot = wave_claim(a.queue);
     bool have = slot < a.n_in;
     uint32_t idx = have ? a.list_in[slot] : 0u;
-    if (have) load_photon(a.ph, idx, L.ph, rng);
+    if (have) load_photon(PH, idx, L.ph, rng);
     lane_reset(L);
     int calls_left = a.budget;
     int calls_done = 0;
@@ -133,7 +138,7 @@ __global__ __launch_bounds__(256, 2) void trace_fp64(TraceLaunch a)
             ++calls_done;
             --calls_left;
             if (L.ph.term || calls_left == 0) {
-                store_photon(a.ph, idx, L.ph, rng, au, av);
+                store_photon(PH, idx, L.ph, rng, au, av);
                 if (L.ph.term) {
                     const uint32_t step = (uint32_t)(a.step0 + calls_done - 1);
                     last_term = step > last_term ? step : last_term;
@@ -145,7 +150,7 @@ __global__ __launch_bounds__(256, 2) void trace_fp64(TraceLaunch a)
                 have = slot < a.n_in;
                 if (have) {
                     idx = a.list_in[slot];
-                    load_photon(a.ph, idx, L.ph, rng);
+                    load_photon(PH, idx, L.ph, rng);
                     lane_reset(L);
                     calls_left = a.budget;
                     calls_done = 0;
@@ -281,7 +286,7 @@ hipError_t launch_init_image(const PhotonPtrs& ph, uint64_t n, uint64_t id0, con
 hipError_t launch_trace(const TraceLaunch& a, int blocks, int block, size_t lds_bytes,
                         hipStream_t stream)
 {
-    if (a.P.integrator == GRT_RK4)
+    if (a.integrator == GRT_RK4)
         hipLaunchKernelGGL(trace_fp64<GRT_RK4>, dim3(blocks), dim3(block), lds_bytes, stream, a);
     else
         hipLaunchKernelGGL(trace_fp64<GRT_RK45>, dim3(blocks), dim3(block), lds_bytes, stream, a);

@@ -18,7 +18,7 @@ import os
 import numpy as np
 
 _PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG_ROOT, "libgrt_amd.so")
+LIB_PATH = os.environ.get("GRT_AMD_LIB") or os.path.join(_PKG_ROOT, "libgrt_amd.so")
 
 GRT_RK4, GRT_RK45 = 0, 1
 MODE_IMAGE, MODE_DISK = 0, 1
