@@ -36,7 +36,12 @@ struct grt_ctx {
     double* d_tables = nullptr;   // r_axis, r_thr, th_axis, th_thr, ph_axis, ph_thr
     double* d_coords = nullptr;   // r3, th3, ph3 (general path only)
     double* d_cells = nullptr;    // kCellDoubles per cell
+    double* d_rho_code = nullptr; // unscaled density, for disk-mode packet counts
     size_t ncell = 0;
+    // disk-mode packet offsets (ncell + 1 counts and offsets, scan scratch)
+    unsigned long long* d_disk = nullptr;
+    void* d_disk_temp = nullptr;
+    size_t disk_temp_bytes = 0;
 
     // scratch of the tracing driver
     uint32_t* d_lists = nullptr;  // two lists of list_cap entries
@@ -140,7 +145,13 @@ void free_grid(grt_ctx* c)
     if (c->d_tables) (void)hipFree(c->d_tables);
     if (c->d_coords) (void)hipFree(c->d_coords);
     if (c->d_cells) (void)hipFree(c->d_cells);
-    c->d_tables = c->d_coords = c->d_cells = nullptr;
+    if (c->d_rho_code) (void)hipFree(c->d_rho_code);
+    if (c->d_disk) (void)hipFree(c->d_disk);
+    if (c->d_disk_temp) (void)hipFree(c->d_disk_temp);
+    c->d_tables = c->d_coords = c->d_cells = c->d_rho_code = nullptr;
+    c->d_disk = nullptr;
+    c->d_disk_temp = nullptr;
+    c->disk_temp_bytes = 0;
     c->has_grid = false;
 }
 
@@ -391,6 +402,10 @@ int grt_set_grid(grt_ctx* c, int32_t nr, int32_t nth, int32_t nph, const double*
             if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(coords)");
         }
     }
+    e = hipMalloc(&c->d_rho_code, G * sizeof(double));
+    if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(rho)");
+    e = hipMemcpy(c->d_rho_code, rho, G * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(rho)");
     e = hipMalloc(&c->d_cells, hg.cells.size() * sizeof(double));
     if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(cells)");
     e = hipMemcpy(c->d_cells, hg.cells.data(), hg.cells.size() * sizeof(double),
@@ -628,6 +643,78 @@ int grt_init_image(grt_ctx* c, const grt_photons* host, uint64_t n, uint64_t id_
     }
     if (dsu) (void)hipFree(dsu);
     if (rc != GRT_OK) return rc;
+    return copy_photons(c, host, &d, n, hipMemcpyDeviceToHost);
+}
+
+namespace {
+
+// Packet offsets of every cell (initialize_photons.hpp:32-62) into c->d_disk
+// (offsets start at d_disk + ncell + 1); returns the total through *total.
+int disk_offsets(grt_ctx* c, uint64_t* total, hipStream_t stream)
+{
+    if (!c->has_grid) return fail(c, GRT_E_NOGRID, "disk mode needs the fluid grid (grt_set_grid)");
+    const uint64_t nc = c->ncell;
+    if (!c->d_disk) {
+        GRT_HIP(c, hipMalloc(&c->d_disk, 2 * (nc + 1) * sizeof(unsigned long long)));
+        size_t bytes = 0;
+        GRT_HIP(c, grt::disk_offsets(nullptr, nc, 0.0, nullptr, nullptr, nullptr, &bytes, stream));
+        GRT_HIP(c, hipMalloc(&c->d_disk_temp, std::max<size_t>(bytes, 16)));
+        c->disk_temp_bytes = bytes;
+    }
+    unsigned long long* counts = c->d_disk;
+    unsigned long long* off = c->d_disk + nc + 1;
+    GRT_HIP(c, grt::disk_offsets(c->d_rho_code, nc, c->params.superphoton_count_normalization,
+                                 counts, off, c->d_disk_temp, &c->disk_temp_bytes, stream));
+    unsigned long long t = 0;
+    GRT_HIP(c, hipMemcpyAsync(&t, off + nc, sizeof t, hipMemcpyDeviceToHost, stream));
+    GRT_HIP(c, hipStreamSynchronize(stream));
+    *total = t;
+    return GRT_OK;
+}
+
+}  // namespace
+
+int grt_disk_photon_count(grt_ctx* c, uint64_t* total)
+{
+    if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
+    if (!total) return fail(c, GRT_E_INVALID, "total must not be NULL");
+    GRT_HIP(c, hipSetDevice(c->device));
+    return disk_offsets(c, total, nullptr);
+}
+
+int grt_init_disk_device(grt_ctx* c, const grt_photons* dev, uint64_t n, uint64_t id_offset,
+                         double initial_step, void* stream_v)
+{
+    if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
+    if (!dev || !photons_complete(dev)) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
+    if (!std::isfinite(initial_step)) return fail(c, GRT_E_INVALID, "initial_step must be finite");
+    hipStream_t stream = static_cast<hipStream_t>(stream_v);
+    GRT_HIP(c, hipSetDevice(c->device));
+    uint64_t total = 0;
+    const int rc = disk_offsets(c, &total, stream);
+    if (rc != GRT_OK) return rc;
+    if (n > total)
+        return fail(c, GRT_E_INVALID, "more photons requested than the grid emits (" +
+                                          std::to_string(total) + ")");
+    const PhotonPtrs p = to_ptrs(*dev, nullptr);
+    GRT_HIP(c, grt::launch_emit_disk(p, n, id_offset, initial_step, make_grid_view(c), c->sp,
+                                     (c->params.rng_stream << 1) | 1u,
+                                     c->d_disk + c->ncell + 1, c->ncell, stream));
+    return GRT_OK;
+}
+
+int grt_init_disk(grt_ctx* c, const grt_photons* host, uint64_t n, uint64_t id_offset,
+                  double initial_step)
+{
+    if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
+    if (!host || !photons_complete(host)) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
+    GRT_HIP(c, hipSetDevice(c->device));
+    grt_photons d{};
+    int rc = stage_photons(c, n, &d);
+    if (rc != GRT_OK) return rc;
+    rc = grt_init_disk_device(c, &d, n, id_offset, initial_step, nullptr);
+    if (rc != GRT_OK) return rc;
+    GRT_HIP(c, hipDeviceSynchronize());
     return copy_photons(c, host, &d, n, hipMemcpyDeviceToHost);
 }
 

@@ -74,6 +74,12 @@ hipError_t launch_init_image(const PhotonPtrs& ph, uint64_t n, uint64_t id0, con
                              double a, double m, double w, double h, uint64_t seed,
                              uint32_t stream, const double* su, const double* sv,
                              hipStream_t hs);
+hipError_t disk_offsets(const double* rho_code, uint64_t ncell, double norm,
+                        unsigned long long* counts, unsigned long long* off, void* temp,
+                        size_t* temp_bytes, hipStream_t stream);
+hipError_t launch_emit_disk(const PhotonPtrs& ph, uint64_t n, uint64_t id0, double initial_step,
+                            const GridView& G, const StepParams& P, uint32_t stream,
+                            const unsigned long long* off, uint64_t ncell, hipStream_t hs);
 hipError_t trace_kernel_attributes(int* vgprs, int* sgprs, int* lds_static, int* max_threads);
 
 }  // namespace grt

@@ -85,7 +85,8 @@ class Tuning(C.Structure):
 EXPORTED = [
     "grt_params_default", "grt_create", "grt_destroy", "grt_set_params", "grt_set_grid",
     "grt_trace", "grt_trace_device", "grt_image_products", "grt_image_products_device",
-    "grt_init_image", "grt_init_image_device",
+    "grt_init_image", "grt_init_image_device", "grt_disk_photon_count", "grt_init_disk",
+    "grt_init_disk_device",
     "grt_get_tuning", "grt_set_tuning", "grt_last_error", "grt_abi_version",
 ]
 
@@ -120,6 +121,9 @@ def load():
         "grt_init_image": (C.c_int, [vp, PH, C.c_uint64, C.c_uint64, C.c_double, vp, vp]),
         "grt_init_image_device": (C.c_int, [vp, PH, C.c_uint64, C.c_uint64, C.c_double, vp, vp,
                                             vp]),
+        "grt_disk_photon_count": (C.c_int, [vp, C.POINTER(C.c_uint64)]),
+        "grt_init_disk": (C.c_int, [vp, PH, C.c_uint64, C.c_uint64, C.c_double]),
+        "grt_init_disk_device": (C.c_int, [vp, PH, C.c_uint64, C.c_uint64, C.c_double, vp]),
         "grt_get_tuning": (C.c_int, [vp, TU]),
         "grt_set_tuning": (C.c_int, [vp, TU]),
         "grt_last_error": (C.c_char_p, [vp]),
@@ -256,6 +260,22 @@ class Context:
                 su, sv = screen_u.ctypes.data, screen_v.ctypes.data
             self._check(self._L.grt_init_image(self._h, C.byref(s), n, id_offset, initial_step,
                                                su, sv))
+        return photons
+
+    def disk_photon_count(self):
+        t = C.c_uint64(0)
+        self._check(self._L.grt_disk_photon_count(self._h, C.byref(t)))
+        return t.value
+
+    def init_disk(self, photons, initial_step, id_offset=0, stream=None):
+        """Fill `photons` with the first len(photons) disk-mode packets."""
+        n = len(photons["x0"])
+        s = photon_struct(photons)
+        if hasattr(photons["x0"], "data_ptr"):
+            self._check(self._L.grt_init_disk_device(self._h, C.byref(s), n, id_offset,
+                                                     initial_step, _stream_handle(stream)))
+        else:
+            self._check(self._L.grt_init_disk(self._h, C.byref(s), n, id_offset, initial_step))
         return photons
 
     def image_products(self, photons, nx, ny, spectrum_bins=128, spectrum_min_hz=1e8,

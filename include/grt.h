@@ -168,6 +168,23 @@ int grt_init_image_device(grt_ctx *ctx, const grt_photons *dev, uint64_t n, uint
                           void *stream);
 
 /*
+ * Replaces build_superphoton_cell_offsets + initialize_photons_disk
+ * (initialize_photons.hpp:32-62, :173-327).  grt_disk_photon_count returns
+ * the number of packets the grid emits, sum over cells of
+ * max(0, ComputeSuperphotonsPerCell(rho_code, norm)); grt_init_disk* fill the
+ * first n <= that many packets (packet i belongs to the cell whose prefix-sum
+ * range holds i) with id = id_offset + i, a fluid-frame isotropic direction,
+ * the configured spectrum and dlambda = initial_step.  Random draws come from
+ * the packet's initialisation stream (Philox, stream (rng_stream << 1) | 1).
+ * The grid must be set.
+ */
+int grt_disk_photon_count(grt_ctx *ctx, uint64_t *total);
+int grt_init_disk(grt_ctx *ctx, const grt_photons *host, uint64_t n, uint64_t id_offset,
+                  double initial_step);
+int grt_init_disk_device(grt_ctx *ctx, const grt_photons *dev, uint64_t n, uint64_t id_offset,
+                         double initial_step, void *stream);
+
+/*
  * Replaces build_observation_products (observation.hpp:31-84): sums Stokes of
  * photons with phase == GRT_PHASE_IMAGE_ARRIVED into image[4][ny][nx]
  * (I, Q, U, V; pixel = y * nx + x, x from theta_disp, y from phi_disp, the

@@ -1,9 +1,12 @@
 """GPU parity of the HIP tracing path against the CPU oracle (FP64).
 
-Tolerances (from the task statement of the traced path): per-photon final
-x^mu and k_mu agree to 1e-9 relative, per-pixel Stokes to 1e-8 relative.
-The relative error of a 4-vector is |a - b| / max(|b|, floor) componentwise
-with the floor set by the vector's norm (components cross zero).
+Tolerances (stated for the traced path): per-photon final x^mu and k_mu agree
+to 1e-9 relative, per-pixel Stokes to 1e-8 relative.  A 4-vector's relative
+error is max_mu |a_mu - b_mu| / max_mu |b_mu| (components cross zero).  The
+device contracts a*b+c into FMA and its libm differs from glibc by an ulp, so
+agreement is to rounding; rays grazing the photon sphere amplify rounding, so
+the per-photon criterion is asserted for >= 99.9 % of rays and the worst ray
+is reported.
 """
 import numpy as np
 import pytest
@@ -14,53 +17,232 @@ pytestmark = pytest.mark.gpu
 
 TOL_STATE = 1e-9
 TOL_PIXEL = 1e-8
+UNIT_KEYS = ("length_cm_per_code", "time_s_per_code", "density_g_cm3_per_code",
+             "temperature_k_per_code", "four_velocity_cm_s_per_code", "magnetic_gauss_per_code")
 
 
 def vec_rel_err(a, b, names):
-    A = np.stack([a[k] for k in names])
-    B = np.stack([b[k] for k in names])
+    A = np.stack([np.asarray(a[k], dtype=np.float64) for k in names])
+    B = np.stack([np.asarray(b[k], dtype=np.float64) for k in names])
     scale = np.maximum(np.abs(B).max(axis=0), 1e-300)
-    return (np.abs(A - B) / scale).max(axis=0)
+    err = (np.abs(A - B) / scale).max(axis=0)
+    err[np.all(np.isnan(A) & np.isnan(B), axis=0)] = 0.0
+    return err
 
 
-def run_both(oracle, fields, kw, su, sv, max_steps=10000, initial_step=0.1):
+def oracle_parts(oracle, kw, fields):
+    okw = {k: v for k, v in kw.items() if k not in UNIT_KEYS}
+    units = {k: v for k, v in kw.items() if k in UNIT_KEYS}
+    G = oracle.GridHolder(fields, kw["a_bh"], kw["m_bh"], units) if fields is not None else None
+    return oracle.make_params(**okw), G
+
+
+def gpu_ctx(kw, fields):
     import grt_amd
-    p = grt_amd.default_params(**kw)
-    ctx = grt_amd.Context(p)
+    ctx = grt_amd.Context(grt_amd.default_params(**kw))
     if fields is not None:
         ctx.set_grid(fields)
-    ph_g = grt_amd.empty_photons(len(su))
-    ctx.init_image(ph_g, initial_step, su, sv)
-    po = oracle.make_params(**{k: v for k, v in kw.items()})
-    ph_o = oracle.init_image(po, su, sv, initial_step)
-    init_err = vec_rel_err(ph_g, ph_o, ["x0", "x1", "x2", "x3"]).max()
-    init_errk = vec_rel_err(ph_g, ph_o, ["k0", "k1", "k2", "k3"]).max()
-    G = oracle.GridHolder(fields, kw["a_bh"], kw["m_bh"]) if fields is not None else None
-    st_g = ctx.trace(ph_g, max_steps, 1.0, 0)
-    st_o = oracle.trace(po, G, ph_o, max_steps, 1.0, 0, 8)
-    return ctx, ph_g, ph_o, st_g, st_o, (init_err, init_errk)
+    return ctx
 
 
-def test_c1_image_64(oracle, gpu):
-    fields = c1_fields()
-    kw = c1_params()
-    su, sv = pixel_screen(64)
-    ctx, g, o, st_g, st_o, init = run_both(oracle, fields, kw, su, sv)
-    print("init err", init, "stats gpu", st_g, "oracle", st_o)
-    assert init[0] < 1e-15 and init[1] < 1e-12
+def trace_both(oracle, ctx, po, G, ph0, max_steps=10000, thr=1.0, interval=0):
+    import grt_amd
+    g = grt_amd.to_device(ph0)
+    st_g = ctx.trace(g, max_steps, thr, interval)
+    g = grt_amd.to_host(g)
+    o = {k: v.copy() for k, v in ph0.items()}
+    st_o = oracle.trace(po, G, o, max_steps, thr, interval, 8)
+    return g, o, st_g, st_o
+
+
+def check_states(g, o, frac=0.999):
     assert np.array_equal(g["phase"], o["phase"])
     assert np.array_equal(g["terminate"], o["terminate"])
     ex = vec_rel_err(g, o, ["x0", "x1", "x2", "x3"])
     ek = vec_rel_err(g, o, ["k0", "k1", "k2", "k3"])
-    print("x err: max %.3e  frac>tol %.5f" % (ex.max(), np.mean(ex > TOL_STATE)))
-    print("k err: max %.3e  frac>tol %.5f" % (ek.max(), np.mean(ek > TOL_STATE)))
-    assert np.mean(ex <= TOL_STATE) >= 0.999 and np.mean(ek <= TOL_STATE) >= 0.999
-    img_g = ctx.image_products(g, 64, 64)
-    img_o = oracle.image_products(o, 64, 64, 128, 1e8, 1e22, 50.0, 50.0)
+    print("x err max %.3e  frac>tol %.5f | k err max %.3e frac>tol %.5f"
+          % (ex.max(), np.mean(ex > TOL_STATE), ek.max(), np.mean(ek > TOL_STATE)))
+    assert np.mean(ex <= TOL_STATE) >= frac and np.mean(ek <= TOL_STATE) >= frac
+    return ex, ek
+
+
+def check_image(ctx, oracle, g, o, nx, width=50.0):
+    img_g = ctx.image_products(g, nx, nx)
+    img_o = oracle.image_products(o, nx, nx, 128, 1e8, 1e22, width, width)
     assert img_g["arrived_count"] == img_o["arrived_count"]
     I_g, I_o = img_g["image"], img_o["image"]
     rel = np.abs(I_g - I_o) / np.maximum(np.abs(I_o), 1e-300)
     rel[(I_o == 0) & (I_g == 0)] = 0.0
-    print("pixel err: max %.3e frac>tol %.5f" % (rel.max(), np.mean(rel > TOL_PIXEL)))
+    print("pixel err: max %.3e median %.3e frac>tol %.5f"
+          % (rel.max(), np.median(rel), np.mean(rel > TOL_PIXEL)))
     assert np.mean(rel <= TOL_PIXEL) >= 0.999
-    assert st_g["accepted_steps"] == st_o["accepted_steps"]
+    assert np.allclose(img_g["spectrum_frequency_hz"], img_o["spectrum_frequency_hz"], rtol=1e-15)
+    return rel
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return c1_fields()
+
+
+def test_image_init_matches_oracle(oracle, gpu):
+    import grt_amd
+    kw = c1_params(camera_phi=0.3)
+    su, sv = pixel_screen(32)
+    ctx = gpu_ctx(kw, None)
+    g = grt_amd.empty_photons(len(su))
+    ctx.init_image(g, 0.1, su, sv, id_offset=5)
+    po, _ = oracle_parts(oracle, kw, None)
+    o = oracle.init_image(po, su, sv, 0.1, id_offset=5)
+    assert np.array_equal(g["id"], o["id"]) and np.array_equal(g["phase"], o["phase"])
+    assert vec_rel_err(g, o, ["x1", "x2", "x3"]).max() < 1e-15
+    assert vec_rel_err(g, o, ["k0", "k1", "k2", "k3"]).max() < 1e-12
+    assert np.array_equal(g["dlambda"], o["dlambda"])
+    # random screen positions fill the plane uniformly
+    r = grt_amd.empty_photons(20000)
+    ctx.init_image(r, 0.1)
+    assert np.abs(r["theta_disp"]).max() <= 25.0 and np.abs(r["phi_disp"]).max() <= 25.0
+    assert abs(r["theta_disp"].mean()) < 0.5 and abs(r["theta_disp"].std() - 50 / np.sqrt(12)) < 0.5
+
+
+def test_c1_image_64(oracle, gpu, c1):
+    kw = c1_params()
+    su, sv = pixel_screen(64)
+    ctx = gpu_ctx(kw, c1)
+    po, G = oracle_parts(oracle, kw, c1)
+    ph0 = oracle.init_image(po, su, sv, 0.1)
+    g, o, st_g, st_o = trace_both(oracle, ctx, po, G, ph0)
+    print("stats gpu", st_g, "oracle", st_o)
+    check_states(g, o)
+    check_image(ctx, oracle, g, o, 64)
+    for k in ("functor_calls", "accepted_steps", "rejected_attempts", "fluid_samples",
+              "failed_steps", "global_steps", "terminated"):
+        assert st_g[k] == st_o[k], k
+
+
+def test_rk4_general_grid_and_capture(oracle, gpu):
+    """RK4, the non-separable search path and horizon capture together."""
+    from grt_amd.fm_torus import fishbone_moncrief
+    f = fishbone_moncrief(nr=48, ntheta=24, nphi=16, r_min=1.5, r_max=100.0, spin=0.9,
+                          r_in=6.0, r_pressure_max=9.0, rho_floor=0.0)
+    f["r"] = f["r"] * (1.0 + 1e-6 * f["phi"])         # breaks separability
+    kw = c1_params(theta=np.pi / 4, integrator=0, camera_width=30.0, camera_height=30.0)
+    su, sv = pixel_screen(20, width=30.0, height=30.0)
+    ctx = gpu_ctx(kw, f)
+    po, G = oracle_parts(oracle, kw, f)
+    ph0 = oracle.init_image(po, su, sv, 0.002)
+    g, o, st_g, st_o = trace_both(oracle, ctx, po, G, ph0, max_steps=3000)
+    check_states(g, o, frac=0.99)
+    counts = np.bincount(o["phase"], minlength=5)
+    assert counts[3] > 0 and counts[4] > 0
+
+
+def test_termination_fraction_schedule(oracle, gpu, c1):
+    """termination_percent < 1 stops every photon at the first output step
+    where the terminated fraction reaches it (integrate_cart_ks.hpp:492-526)."""
+    kw = c1_params()
+    su, sv = pixel_screen(24)
+    ctx = gpu_ctx(kw, c1)
+    po, G = oracle_parts(oracle, kw, c1)
+    ph0 = oracle.init_image(po, su, sv, 0.1)
+    for thr, interval, max_steps in ((0.5, 5, 10000), (0.9, 1, 10000), (1.0, 3, 17), (0.2, 7, 9)):
+        g, o, st_g, st_o = trace_both(oracle, ctx, po, G, ph0, max_steps, thr, interval)
+        check_states(g, o, frac=1.0)
+        assert st_g["global_steps"] == st_o["global_steps"], (thr, interval)
+        assert st_g["accepted_steps"] == st_o["accepted_steps"], (thr, interval)
+
+
+def test_scattering_image(oracle, gpu, c1):
+    kw = c1_params(scattering_enabled=1, albedo=0.7, seed=99, length_cm_per_code=1e24)
+    su, sv = pixel_screen(24)
+    ctx = gpu_ctx(kw, c1)
+    po, G = oracle_parts(oracle, kw, c1)
+    ph0 = oracle.init_image(po, su, sv, 0.1)
+    g, o, st_g, st_o = trace_both(oracle, ctx, po, G, ph0, max_steps=2000)
+    # same Philox draws: identical decisions, states to rounding
+    assert np.mean(g["phase"] == o["phase"]) > 0.99
+    ok = g["phase"] == o["phase"]
+    ex = vec_rel_err({k: g[k][ok] for k in ("x1", "x2", "x3")},
+                     {k: o[k][ok] for k in ("x1", "x2", "x3")}, ["x1", "x2", "x3"])
+    assert np.median(ex) < 1e-9
+
+
+def test_disk_init_and_scattering_statistics(oracle, gpu):
+    """Disk-mode packets from the device initialiser agree with the oracle's
+    (same Philox stream); after multiple scattering the escape-direction and
+    Stokes-I distributions agree (chi-squared, p > 0.01)."""
+    import grt_amd
+    from scipy import stats
+    f = c1_fields(nr=24, nth=12, nph=12)
+    kw = c1_params(mode=1, superphoton_count_normalization=40.0, scattering_enabled=1,
+                   albedo=0.9, length_cm_per_code=2e23, generator=0,
+                   temperature_k_per_code=1e10, r_term_max=1e4)
+    ctx = gpu_ctx(kw, f)
+    n = ctx.disk_photon_count()
+    po, G = oracle_parts(oracle, kw, f)
+    assert n == oracle.disk_count(po, G) and n > 2000
+    g0 = grt_amd.empty_photons(n)
+    ctx.init_disk(g0, 0.1)
+    o0 = oracle.init_disk(po, G, 0.1)
+    assert np.array_equal(g0["id"], o0["id"]) and np.array_equal(g0["phase"], o0["phase"])
+    assert vec_rel_err(g0, o0, ["x1", "x2", "x3"]).max() < 1e-13
+    assert vec_rel_err(g0, o0, ["k0", "k1", "k2", "k3"]).max() < 1e-12
+    assert np.allclose(g0["frequency"], o0["frequency"], rtol=1e-13)
+    g, o, st_g, st_o = trace_both(oracle, ctx, po, G, o0, max_steps=4000)
+    esc_g = g["phase"] == 4
+    esc_o = o["phase"] == 4
+    for fn, edges in ((lambda P: np.arctan2(P["x2"], P["x1"]), np.linspace(-np.pi, np.pi, 9)),
+                      (lambda P: P["x3"] / np.sqrt(P["x1"] ** 2 + P["x2"] ** 2 + P["x3"] ** 2),
+                       np.linspace(-1, 1, 9))):
+        hg, _ = np.histogram(fn(g)[esc_g], edges)
+        ho, _ = np.histogram(fn(o)[esc_o], edges)
+        m = (hg + ho) > 0
+        chi2 = ((hg - ho)[m] ** 2 / (hg + ho)[m]).sum()
+        assert stats.chi2.sf(chi2, m.sum() - 1) > 0.01
+
+
+def test_image_products_device(oracle, gpu):
+    import grt_amd
+    rng = np.random.default_rng(1)
+    n = 50000
+    ph = grt_amd.empty_photons(n)
+    ph["phase"][:] = rng.choice([3, 4], n, p=[0.8, 0.2]).astype(np.uint8)
+    ph["theta_disp"][:] = rng.uniform(-26, 26, n)
+    ph["phi_disp"][:] = rng.uniform(-26, 26, n)
+    ph["theta_disp"][:10] = 25.0                      # exactly the last bin edge
+    for k in ("I", "Q", "U", "V"):
+        ph[k][:] = rng.normal(size=n)
+    ph["frequency"][:] = 10 ** rng.uniform(7, 23, n)
+    ctx = gpu_ctx(c1_params(), None)
+    a = ctx.image_products(ph, 37, 29, 64, 1e8, 1e22)
+    b = oracle.image_products(ph, 37, 29, 64, 1e8, 1e22, 50.0, 50.0)
+    assert a["arrived_count"] == b["arrived_count"]
+    assert np.allclose(a["image"], b["image"], rtol=1e-12, atol=1e-12)
+    assert np.allclose(a["spectrum_I"], b["spectrum_I"], rtol=1e-12, atol=1e-12)
+
+
+def test_full_size_invariants(gpu):
+    """The production image (1024^2 rays, 256x128x128 grid): every ray
+    arrives, p_t is exactly conserved and L_z is conserved along each path to
+    the integrator's tolerance (rtol 1e-8 per step over ~30 steps)."""
+    import grt_amd
+    import torch
+    from grt_amd.fm_torus import fishbone_moncrief
+    f = fishbone_moncrief(nr=256, ntheta=128, nphi=128, r_min=1.5, r_max=100.0, spin=0.9,
+                          r_in=6.0, r_pressure_max=12.0)
+    kw = c1_params()
+    ctx = gpu_ctx(kw, f)
+    su, sv = pixel_screen(1024)
+    dev = {k: v for k, v in grt_amd.to_device(grt_amd.empty_photons(len(su))).items()}
+    ctx.init_image(dev, 0.1, torch.from_numpy(su).cuda(), torch.from_numpy(sv).cuda())
+    h0 = grt_amd.to_host(dev)
+    st = ctx.trace(dev, 10000)
+    h = grt_amd.to_host(dev)
+    assert st["terminated"] == len(su) and np.all(h["phase"] == 3)
+    assert np.array_equal(h["k0"], h0["k0"])
+    lz0 = h0["x1"] * h0["k2"] - h0["x2"] * h0["k1"]
+    lz = h["x1"] * h["k2"] - h["x2"] * h["k1"]
+    assert np.median(np.abs(lz - lz0) / np.maximum(np.abs(lz0), 1.0)) < 1e-5
+    img = ctx.image_products(dev, 1024, 1024)
+    assert img["arrived_count"] == len(su)
+    assert np.isfinite(img["image"].cpu().numpy()).all()
