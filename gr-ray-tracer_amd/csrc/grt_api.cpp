@@ -33,7 +33,8 @@ struct grt_ctx {
     bool has_grid = false;
     int nr = 0, nth = 0, nph = 0;
     bool separable = false;
-    double* d_tables = nullptr;   // r_axis, r_thr, th_axis, th_thr, ph_axis, ph_thr
+    double* d_tables = nullptr;   // separable search tables, then the three axes
+    size_t table_doubles = 0;
     double* d_coords = nullptr;   // r3, th3, ph3 (general path only)
     double* d_cells = nullptr;    // kCellDoubles per cell
     double* d_rho_code = nullptr; // unscaled density, for disk-mode packet counts
@@ -388,10 +389,12 @@ int grt_set_grid(grt_ctx* c, int32_t nr, int32_t nth, int32_t nph, const double*
     c->separable = hg.separable;
     hipError_t e;
     if (hg.separable) {
-        e = hipMalloc(&c->d_tables, hg.tables.size() * sizeof(double));
+        std::vector<double> t = hg.tables;
+        t.insert(t.end(), hg.axes.begin(), hg.axes.end());
+        c->table_doubles = hg.tables.size();
+        e = hipMalloc(&c->d_tables, t.size() * sizeof(double));
         if (e != hipSuccess) return hip_fail(c, e, "hipMalloc(tables)");
-        e = hipMemcpy(c->d_tables, hg.tables.data(), hg.tables.size() * sizeof(double),
-                      hipMemcpyHostToDevice);
+        e = hipMemcpy(c->d_tables, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice);
         if (e != hipSuccess) return hip_fail(c, e, "hipMemcpy(tables)");
     } else {
         e = hipMalloc(&c->d_coords, 3 * G * sizeof(double));
@@ -420,7 +423,8 @@ namespace {
 GridView make_grid_view(const grt_ctx* c)
 {
     return grt::grid_view(c->nr, c->nth, c->nph, c->has_grid, c->separable, c->d_tables,
-                          c->d_coords, c->d_cells);
+                          c->d_tables ? c->d_tables + c->table_doubles : nullptr, c->d_coords,
+                          c->d_cells);
 }
 
 }  // namespace
@@ -470,7 +474,7 @@ int grt_trace_device(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max
     L.queue = c->d_counters + 0;
     L.n_out = c->d_counters + 1;
     L.stats = c->d_stats;
-    const int lds_doubles = (c->has_grid && c->separable) ? 2 * (c->nr + c->nth + c->nph) : 0;
+    const int lds_doubles = (c->has_grid && c->separable) ? (int)c->table_doubles : 0;
     L.use_lds = lds_doubles > 0 ? 1 : 0;
     L.lds_doubles = lds_doubles;
     const size_t lds_bytes = (size_t)lds_doubles * sizeof(double);

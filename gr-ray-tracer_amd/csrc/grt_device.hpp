@@ -5,9 +5,10 @@
 // the host to cross-check the kernel's logic against the oracle.
 //
 // Arithmetic follows the reference's expressions term by term (same
-// operations, same order; the device compiler may fuse a*b+c into FMA), so the
-// FP64 path agrees with the reference to rounding.  Citations are file:line in
-// the reference's src/ directory.
+// operations, same order) and the library is compiled without FMA
+// contraction, so +, -, *, / and sqrt round exactly as in the reference; the
+// step controller's x^(1/4) is correctly rounded (quarter_root).  Citations
+// are file:line in the reference's src/ directory.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -95,6 +96,18 @@ GRT_HD void ks_metric(const double X[4], double a, double m, double g[16])
     }
 }
 
+// n / d correctly rounded, given y = 1/d correctly rounded: q = n y is within
+// an ulp of n/d, its residual n - q d is exact with fma, and q + residual * y
+// rounds to the IEEE quotient (Markstein's theorem for binary floating point;
+// see Muller et al., Handbook of Floating-Point Arithmetic, ch. 5).  Valid
+// away from overflow/underflow, which the geometry here never approaches.
+GRT_HD double div_rn(double n, double d, double y)
+{
+    const double q = n * y;
+    const double res = fma(-q, d, n);
+    return fma(res, y, q);
+}
+
 // Hamilton's equations for H = 1/2 g^{mu nu} p_mu p_nu in Cartesian
 // Kerr-Schild form, kerr_schild_core.hpp:161-192 (with r, dr/dx^k of :36-46,
 // l, dl/dx^k of :59-77 and H, dH/dx^k of :86-95 expanded for k = x, y, z).
@@ -131,11 +144,17 @@ GRT_HD void ks_rhs(const double s[8], double a, double m, double out[8])
     const double drz = (z * f + 2.0 * a2 * z * iS) * itr;
 #define GRT_DIV(n, d, inv) ((n) * (inv))
 #else
-    const double f = 1.0 + A / S;
-    const double drx = x * f / tr;
-    const double dry = y * f / tr;
-    const double drz = (z * f + 2.0 * a2 * z / S) / tr;
-#define GRT_DIV(n, d, inv) ((n) / (d))
+    // Every quotient n/d is the correctly rounded IEEE quotient, evaluated
+    // from the correctly rounded reciprocal y = 1/d of its divisor with
+    // Markstein's correction (div_rn), so the five divisors cost one
+    // division each instead of one per quotient.  1/(2r) = (1/r)/2 exactly.
+    const double iS = 1.0 / S, ir = 1.0 / r, iB = 1.0 / B, iD = 1.0 / D, iD2 = 1.0 / D2;
+    const double itr = 0.5 * ir;
+    const double f = 1.0 + div_rn(A, S, iS);
+    const double drx = div_rn(x * f, tr, itr);
+    const double dry = div_rn(y * f, tr, itr);
+    const double drz = div_rn(z * f + div_rn(2.0 * a2 * z, S, iS), tr, itr);
+#define GRT_DIV(n, d, inv) div_rn((n), (d), (inv))
 #endif
 
     const double l1 = GRT_DIV(r * x + a * y, B, iB);
@@ -192,6 +211,32 @@ constexpr double A61 = -8.0 / 27.0, A62 = 2.0, A63 = -3544.0 / 2565.0, A64 = 185
 constexpr double B41 = 25.0 / 216.0, B43 = 1408.0 / 2565.0, B44 = 2197.0 / 4104.0, B45 = -1.0 / 5.0;
 constexpr double B51 = 16.0 / 135.0, B53 = 6656.0 / 12825.0, B54 = 28561.0 / 56430.0,
                  B55 = -9.0 / 50.0, B56 = 2.0 / 55.0;
+
+// x^(1/4) correctly rounded: sqrt(sqrt(x)) is within an ulp; of it and its two
+// neighbours keep the one whose fourth power is closest to x, evaluated with
+// exact products (fma).  The step controller's pow(1/err, 0.25)
+// (integrate_cart_ks.hpp:217) is a libm call whose last bit varies between
+// libraries; using the correctly rounded value in both this kernel and the
+// oracle makes the step-size sequence reproducible bit for bit.
+GRT_HD double quarter_root(double x)
+{
+    const double y = sqrt(sqrt(x));
+    double best = y, bestd = 1.0 / 0.0;
+#pragma unroll
+    for (int d = -1; d <= 1; ++d) {
+        const double c = d < 0 ? nextafter(y, 0.0) : (d > 0 ? nextafter(y, 1.0 / 0.0) : y);
+        const double s = c * c;
+        const double e1 = fma(c, c, -s);
+        const double q = s * s;
+        const double e2 = fma(s, s, -q);
+        const double r = fabs((q - x) + (e2 + 2.0 * s * e1));
+        if (r < bestd) {
+            bestd = r;
+            best = c;
+        }
+    }
+    return best;
+}
 
 // Component 4 (p_t) has a zero derivative, so every stage leaves it equal to
 // s[4]; the loops below skip it.
@@ -277,7 +322,7 @@ GRT_HD bool rk45_attempt(double s[8], double* h_io, const double k1[8], const St
     if (err == 0.0) {
         scale = P.smax;
     } else {
-        scale = P.safety * pow(1.0 / err, 0.25);
+        scale = P.safety * quarter_root(1.0 / err);
         scale = fmin(P.smax, fmax(P.smin, scale));
     }
     *h_io = h * scale;
@@ -303,13 +348,11 @@ struct GridView {
     int nr, nth, nph;
     int available;
     int separable;            // 1: coordinates depend only on their own index
-    // separable fast path: 1-D axes and 0.500001 * neighbour span per index
-    const double* r_axis;
-    const double* r_thr;
-    const double* th_axis;
-    const double* th_thr;
-    const double* ph_axis;
-    const double* ph_thr;
+    // separable fast path: search tables, 4 * (nr + nth + nph) doubles
+    // (layout and meaning: see locate_separable)
+    const double* tab;
+    // the 1-D axes r_i, theta_j, phi_k of a separable grid (cell centres)
+    const double* axes;
     // general path: the full (nr, nth, nph) coordinate fields
     const double* r3;
     const double* th3;
@@ -317,10 +360,12 @@ struct GridView {
     const double* cells;      // kCellDoubles per cell, C order
 };
 
-// scattering.hpp:41-45
+// scattering.hpp:41-45.  fmod(d, 2 pi) is d itself for 0 <= d < 2 pi, the
+// only range the arguments here reach; the library call is kept for the rest.
 GRT_HD double periodic_distance(double lhs, double rhs)
 {
-    const double d = fmod(fabs(lhs - rhs), kTwoPi);
+    double d = fabs(lhs - rhs);
+    if (!(d < kTwoPi)) d = fmod(d, kTwoPi);
     return fmin(d, kTwoPi - d);
 }
 
@@ -340,82 +385,112 @@ GRT_HD int count_le(const double* t, int n, double v)
     return lo;
 }
 
+// Number of leading entries >= v in the descending table t[0..n).
+GRT_HD int count_ge_desc(const double* t, int n, double v)
+{
+    int lo = 0, len = n;
+    while (len > 0) {
+        const int half = len >> 1;
+        if (t[lo + half] >= v) {
+            lo += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    return lo;
+}
+
+// A monotone stand-in for atan2(Y, X) mapped to [0, 2 pi): the "diamond
+// angle" in [0, 4), one division, no transcendental.
+GRT_HD double pseudo_angle(double X, double Y)
+{
+    if (Y >= 0.0) {
+        if (X >= 0.0) return (X + Y > 0.0) ? Y / (X + Y) : 0.0;
+        return 1.0 + (-X) / (Y - X);
+    }
+    if (X < 0.0) return 2.0 + (-Y) / (-X - Y);
+    return 3.0 + X / (X - Y);
+}
+
 // Separable fast path of locate_structured_grid_cell (scattering.hpp:151-180).
-// With r(i,j,k) = r(i), theta(i,j,k) = theta(j), phi(i,j,k) = phi(k), the
-// three coordinate-descent passes converge in the first, and every
-// nearest_* scan (first strict minimum of the same distance expression) is
-// resolved by a binary search plus the bracketing candidates: |log(r/c)| and
-// |theta - c| are V-shaped in the index, and the periodic azimuth distance is
-// minimised at the bracket or at either end of the axis.  The host only
-// enables this path for positive, strictly ascending axes with phi in [0, 2pi).
-GRT_HD bool locate_separable(const GridView& G, double radius, double polar, double az,
+// With r(i,j,k) = r(i), theta(i,j,k) = theta(j), phi(i,j,k) = phi(k) the three
+// coordinate-descent passes converge in the first, and each nearest_* scan
+// (first strict minimum) reduces to a bracket search plus the candidates
+// around it.  Every comparison of the reference is replaced by an equivalent
+// one on quantities the host tabulated, so no log, acos, atan2 or fmod runs
+// here (decisions can differ from the reference's only when a point lies
+// within rounding of a cell boundary or midpoint):
+//   radial   |log(r/c_lo)| vs |log(r/c_hi)|  <=>  r vs sqrt(c_lo c_hi)
+//            |log(r/c_i)| <= thr_i          <=>  c_i e^-thr_i <= r <= c_i e^thr_i
+//   polar    theta = acos(ct); theta_j <= theta <=> cos theta_j >= ct, the
+//            midpoint and the +-thr_j window mapped through cos
+//   azimuth  periodic distance to phi_k is decreasing in the dot product of
+//            (X, Y) = (r x + a y, r y - a x) with (cos phi_k, sin phi_k); the
+//            bracket comes from a monotone pseudo-angle; the window test is
+//            dot >= cos(thr_k) |(X, Y)|
+// Table layout (doubles), n = nr, m = nth, q = nph:
+//   [0,n) r_j   [n,2n) sqrt(r_j r_j+1)   [2n,3n) r_j e^-thr   [3n,4n) r_j e^thr
+//   then cos th_j | cos of midpoints | cos(min(pi, th_j + thr)) | cos(max(0, th_j - thr))
+//   then pseudo(phi_k) | cos phi_k | sin phi_k | cos(thr_k) (-2: single entry)
+GRT_HD bool locate_separable(const GridView& G, double radius, double ct, double X, double Y,
                              int& i_out, int& j_out, int& k_out)
 {
+    const int nr = G.nr, nth = G.nth, nph = G.nph;
+    const double* R = G.tab;
+    const double* T = R + 4 * nr;
+    const double* F = T + 4 * nth;
     // radial (scattering.hpp:47-65)
     int i;
-    double di;
     {
-        const int lo = count_le(G.r_axis, G.nr, radius) - 1;
-        if (lo < 0) {
-            i = 0;
-            di = fabs(log(radius / G.r_axis[0]));
-        } else if (lo >= G.nr - 1) {
-            i = G.nr - 1;
-            di = fabs(log(radius / G.r_axis[G.nr - 1]));
-        } else {
-            const double d0 = fabs(log(radius / G.r_axis[lo]));
-            const double d1 = fabs(log(radius / G.r_axis[lo + 1]));
-            i = (d1 < d0) ? lo + 1 : lo;
-            di = (d1 < d0) ? d1 : d0;
-        }
+        const int lo = count_le(R, nr, radius) - 1;
+        if (lo < 0) i = 0;
+        else if (lo >= nr - 1) i = nr - 1;
+        else i = (radius > R[nr + lo]) ? lo + 1 : lo;
     }
     // polar (scattering.hpp:67-81)
     int j;
-    double dj;
     {
-        const int lo = count_le(G.th_axis, G.nth, polar) - 1;
-        if (lo < 0) {
-            j = 0;
-            dj = fabs(polar - G.th_axis[0]);
-        } else if (lo >= G.nth - 1) {
-            j = G.nth - 1;
-            dj = fabs(polar - G.th_axis[G.nth - 1]);
-        } else {
-            const double d0 = fabs(polar - G.th_axis[lo]);
-            const double d1 = fabs(polar - G.th_axis[lo + 1]);
-            j = (d1 < d0) ? lo + 1 : lo;
-            dj = (d1 < d0) ? d1 : d0;
-        }
+        const int lo = count_ge_desc(T, nth, ct) - 1;
+        if (lo < 0) j = 0;
+        else if (lo >= nth - 1) j = nth - 1;
+        else j = (ct < T[nth + lo]) ? lo + 1 : lo;
     }
-    // azimuth (scattering.hpp:83-97): candidates 0, lo, lo+1, n-1 in index order
+    // azimuth (scattering.hpp:83-97): candidates 0, lo, lo+1, nph-1
     int k = 0;
-    double dk = periodic_distance(az, G.ph_axis[0]);
+    double best = X * F[nph] + Y * F[2 * nph];
     {
-        const int n = G.nph;
-        const int lo = count_le(G.ph_axis, n, az) - 1;
-        const int c1 = lo, c2 = lo + 1, c3 = n - 1;
-        if (c1 > 0 && c1 < n) {
-            const double d = periodic_distance(az, G.ph_axis[c1]);
-            if (d < dk) { dk = d; k = c1; }
+        const int lo = count_le(F, nph, pseudo_angle(X, Y)) - 1;
+        const int c1 = lo, c2 = lo + 1, c3 = nph - 1;
+        if (c1 > 0 && c1 < nph) {
+            const double d = X * F[nph + c1] + Y * F[2 * nph + c1];
+            if (d > best) { best = d; k = c1; }
         }
-        if (c2 > 0 && c2 > c1 && c2 < n) {
-            const double d = periodic_distance(az, G.ph_axis[c2]);
-            if (d < dk) { dk = d; k = c2; }
+        if (c2 > 0 && c2 > c1 && c2 < nph) {
+            const double d = X * F[nph + c2] + Y * F[2 * nph + c2];
+            if (d > best) { best = d; k = c2; }
         }
         if (c3 > 0 && c3 > c2) {
-            const double d = periodic_distance(az, G.ph_axis[c3]);
-            if (d < dk) { dk = d; k = c3; }
+            const double d = X * F[nph + c3] + Y * F[2 * nph + c3];
+            if (d > best) { best = d; k = c3; }
         }
     }
     i_out = i;
     j_out = j;
     k_out = k;
-    // within_* tests (scattering.hpp:99-149); thr = 0.500001 * neighbour span,
-    // or -1 for a single-entry axis whose test is distance <= 1e-10.
-    const double ti = G.r_thr[i], tj = G.th_thr[j], tk = G.ph_thr[k];
-    const bool wi = (ti < 0.0) ? (di <= 1.0e-10) : (di <= ti);
-    const bool wj = (tj < 0.0) ? (dj <= 1.0e-10) : (dj <= tj);
-    const bool wk = (tk < 0.0) ? (dk <= 1.0e-10) : (dk <= tk);
+    // within_* tests (scattering.hpp:99-149)
+    const bool wi = R[2 * nr + i] <= radius && radius <= R[3 * nr + i];
+    const bool wj = T[2 * nth + j] <= ct && ct <= T[3 * nth + j];
+    const double norm = sqrt(X * X + Y * Y);
+    const double cthr = F[3 * nph + k];
+    bool wk;
+    if (cthr >= -1.0) {
+        wk = best >= cthr * norm;
+    } else {
+        // single azimuth: |sin d| <= 1e-10 on the near side
+        const double cross = X * F[2 * nph + k] - Y * F[nph + k];
+        wk = best >= 0.0 && fabs(cross) <= 1.0e-10 * norm;
+    }
     return wi && wj && wk;
 }
 
@@ -565,6 +640,8 @@ GRT_HD bool fluid_tetrad(const KSMetric& g, const double u[4], double e[16])
 
 // sample_scattering_fluid, scattering.hpp:185-251.  `g` receives the metric at
 // the photon position (reused by the scattering update).
+// kSep selects the separable fast search at compile time.
+template <int kSep>
 GRT_HD bool sample_fluid(const double s[8], const GridView& G, const StepParams& P,
                          FluidCellCGS& fl, double e[16], KSMetric& g)
 {
@@ -572,12 +649,18 @@ GRT_HD bool sample_fluid(const double s[8], const GridView& G, const StepParams&
     const double radius = ks_radius(s[1], s[2], s[3], P.a);
     if (!(radius > 0.0) || !is_finite(radius)) return false;
     const double ct = fmax(-1.0, fmin(1.0, s[3] / radius));
-    const double polar = acos(ct);
-    double az = atan2(radius * s[2] - P.a * s[1], radius * s[1] + P.a * s[2]);
-    if (az < 0.0) az += kTwoPi;
+    const double X = radius * s[1] + P.a * s[2];
+    const double Y = radius * s[2] - P.a * s[1];
     int i, j, k;
-    const bool found = G.separable ? locate_separable(G, radius, polar, az, i, j, k)
-                                   : locate_general(G, radius, polar, az, i, j, k);
+    bool found;
+    if (kSep) {
+        found = locate_separable(G, radius, ct, X, Y, i, j, k);
+    } else {
+        const double polar = acos(ct);
+        double az = atan2(Y, X);
+        if (az < 0.0) az += kTwoPi;
+        found = locate_general(G, radius, polar, az, i, j, k);
+    }
     if (!found) return false;
     const double* c = G.cells + (((size_t)i * G.nth + j) * G.nph + k) * kCellDoubles;
     fl.position_cm[0] = s[1] * P.length_scale;
@@ -759,6 +842,7 @@ struct Photon {
 // phase the call started with; `taken` the affine step just integrated.
 // On arrival at the camera (phase becomes kArrived) the screen coordinates
 // are returned in (au, av).  Returns whether the fluid was sampled.
+template <int kSep>
 GRT_HD bool post_step(Photon& ph, uint8_t phase0, double taken, const GridView& G,
                       const StepParams& P, PhotonRng& rng, double& au, double& av)
 {
@@ -767,7 +851,7 @@ GRT_HD bool post_step(Photon& ph, uint8_t phase0, double taken, const GridView& 
     FluidCellCGS fl;
     double e[16];
     KSMetric g;
-    const bool sampled = needs && sample_fluid(ph.s, G, P, fl, e, g);
+    const bool sampled = needs && sample_fluid<kSep>(ph.s, G, P, fl, e, g);
 
     if (image && phase0 == kBackward && sampled) {
         double nu = 0.0, energy = 0.0;
@@ -927,7 +1011,7 @@ GRT_HD void lane_reset(Lane& L)
 // is thereby split at its retry loop, so lanes whose step is rejected retry
 // while their neighbours go on.  Returns true when the functor call
 // completed in this iteration; (au, av) are set when the photon arrived.
-template <int kIntegrator>
+template <int kIntegrator, int kSep>
 GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, PhotonRng& rng,
                          LaneCounters& n, double& au, double& av)
 {
@@ -940,7 +1024,7 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
             const double taken = ph.dl;
             rk4_step(ph.s, taken, P.a, P.m);
             ++n.accepted;
-            n.samples += post_step(ph, L.phase0, taken, G, P, rng, au, av) ? 1u : 0u;
+            n.samples += post_step<kSep>(ph, L.phase0, taken, G, P, rng, au, av) ? 1u : 0u;
             return true;
         }
         ks_rhs(ph.s, P.a, P.m, L.k1);
@@ -958,7 +1042,7 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
         // register allocator need not keep it alive across the post-step.
 #pragma unroll
         for (int q = 0; q < 8; ++q) L.k1[q] = 0.0;
-        n.samples += post_step(ph, L.phase0, taken, G, P, rng, au, av) ? 1u : 0u;
+        n.samples += post_step<kSep>(ph, L.phase0, taken, G, P, rng, au, av) ? 1u : 0u;
         return true;
     }
     ++n.rejected;

@@ -35,9 +35,9 @@ __device__ __forceinline__ void cell_coords(const GridView& G, uint64_t c, doubl
         const int i = (int)(c / plane);
         const int j = (int)((c / G.nph) % G.nth);
         const int k = (int)(c % G.nph);
-        r = G.r_axis[i];
-        th = G.th_axis[j];
-        ph = G.ph_axis[k];
+        r = G.axes[i];
+        th = G.axes[G.nr + j];
+        ph = G.axes[G.nr + G.nth + k];
     } else {
         r = G.r3[c];
         th = G.th3[c];

@@ -86,12 +86,87 @@ void host_parallel_for(size_t n, F&& f)
 struct HostGrid {
     int nr = 0, nth = 0, nph = 0;
     bool separable = false;
-    std::vector<double> tables;   // r_axis, r_thr, th_axis, th_thr, ph_axis, ph_thr
+    std::vector<double> tables;   // separable search tables, see locate_separable
+    std::vector<double> axes;     // r_i | theta_j | phi_k of a separable grid
     std::vector<double> cells;    // kCellDoubles per cell
 };
 
 // Largest separable table set staged in LDS (bytes).
 constexpr size_t kMaxTableBytes = 48 * 1024;
+
+// The diamond pseudo-angle of (cos phi, sin phi), as pseudo_angle() on device.
+inline double host_pseudo_angle(double X, double Y)
+{
+    if (Y >= 0.0) {
+        if (X >= 0.0) return (X + Y > 0.0) ? Y / (X + Y) : 0.0;
+        return 1.0 + (-X) / (Y - X);
+    }
+    if (X < 0.0) return 2.0 + (-Y) / (-X - Y);
+    return 3.0 + X / (X - Y);
+}
+
+// Builds the separable search tables of locate_separable from the 1-D axes;
+// returns false if the axes do not admit the fast path.
+inline bool build_search_tables(const std::vector<double>& ra, const std::vector<double>& ta,
+                                const std::vector<double>& pa, std::vector<double>& tab)
+{
+    const int nr = (int)ra.size(), nth = (int)ta.size(), nph = (int)pa.size();
+    const double pi = 3.14159265358979323846, two_pi = 2.0 * pi;
+    for (int i = 0; i < nr; ++i)
+        if (!(ra[i] > 0.0) || !std::isfinite(ra[i]) || (i > 0 && !(ra[i] > ra[i - 1]))) return false;
+    for (int j = 0; j < nth; ++j)
+        if (!(ta[j] >= 0.0) || !(ta[j] <= pi) || (j > 0 && !(ta[j] > ta[j - 1]))) return false;
+    for (int k = 0; k < nph; ++k)
+        if (!(pa[k] >= 0.0) || !(pa[k] < two_pi) || (k > 0 && !(pa[k] > pa[k - 1]))) return false;
+    if ((size_t)4 * (nr + nth + nph) * sizeof(double) > kMaxTableBytes) return false;
+    tab.assign(4 * (size_t)(nr + nth + nph), 0.0);
+    const double inf = 1.0 / 0.0;
+    // radial: axis, geometric midpoints, e^-+thr window (scattering.hpp:99-117)
+    double* R = tab.data();
+    for (int i = 0; i < nr; ++i) {
+        double span = 0.0;
+        if (i > 0) span = std::fmax(span, std::fabs(std::log(ra[i] / ra[i - 1])));
+        if (i + 1 < nr) span = std::fmax(span, std::fabs(std::log(ra[i] / ra[i + 1])));
+        const double thr = nr == 1 ? 1.0e-10 : 0.500001 * span;
+        R[i] = ra[i];
+        R[nr + i] = i + 1 < nr ? std::sqrt(ra[i] * ra[i + 1]) : inf;
+        R[2 * nr + i] = ra[i] * std::exp(-thr);
+        R[3 * nr + i] = ra[i] * std::exp(thr);
+    }
+    // polar in cos space (scattering.hpp:119-133)
+    double* T = R + 4 * nr;
+    for (int j = 0; j < nth; ++j) {
+        double span = 0.0;
+        if (j > 0) span = std::fmax(span, std::fabs(ta[j] - ta[j - 1]));
+        if (j + 1 < nth) span = std::fmax(span, std::fabs(ta[j] - ta[j + 1]));
+        const double thr = nth == 1 ? 1.0e-10 : 0.500001 * span;
+        T[j] = std::cos(ta[j]);
+        T[nth + j] = j + 1 < nth ? std::cos(0.5 * (ta[j] + ta[j + 1])) : -inf;
+        T[2 * nth + j] = std::cos(std::fmin(pi, ta[j] + thr));
+        T[3 * nth + j] = std::cos(std::fmax(0.0, ta[j] - thr));
+    }
+    for (int j = 1; j < nth; ++j)
+        if (!(T[j] < T[j - 1])) return false;
+    // azimuth: pseudo-angle, cos, sin, cos(window) (scattering.hpp:135-149)
+    double* F = T + 4 * nth;
+    auto per = [&](double a, double b) {
+        const double d = std::fmod(std::fabs(a - b), two_pi);
+        return std::fmin(d, two_pi - d);
+    };
+    for (int k = 0; k < nph; ++k) {
+        double span = 0.0;
+        if (k > 0) span = std::fmax(span, per(pa[k], pa[k - 1]));
+        if (k + 1 < nph) span = std::fmax(span, per(pa[k], pa[k + 1]));
+        const double c = std::cos(pa[k]), sn = std::sin(pa[k]);
+        F[k] = host_pseudo_angle(c, sn);
+        F[nph + k] = c;
+        F[2 * nph + k] = sn;
+        F[3 * nph + k] = nph == 1 ? -2.0 : std::cos(std::fmin(pi, 0.500001 * span));
+    }
+    for (int k = 1; k < nph; ++k)
+        if (!(F[k] > F[k - 1])) return false;
+    return true;
+}
 
 inline void prepare_grid(const grt_params& p, int nr, int nth, int nph, const double* r,
                          const double* theta, const double* phi, const double* rho,
@@ -104,9 +179,7 @@ inline void prepare_grid(const grt_params& p, int nr, int nth, int nph, const do
     auto at = [&](const double* a, int i, int j, int k) {
         return a[((size_t)i * nth + j) * nph + k];
     };
-    // Separable axes: r(i,j,k) = r(i), theta = theta(j), phi = phi(k), r > 0,
-    // strictly ascending, phi in [0, 2 pi), tables small enough for LDS.
-    const double two_pi = 2.0 * 3.14159265358979323846;
+    // Separable axes: r(i,j,k) = r(i), theta = theta(j), phi = phi(k).
     bool sep = true;
     for (size_t c = 0; c < G && sep; ++c) {
         const int i = (int)(c / ((size_t)nth * nph));
@@ -119,51 +192,14 @@ inline void prepare_grid(const grt_params& p, int nr, int nth, int nph, const do
     for (int i = 0; i < nr; ++i) ra[i] = at(r, i, 0, 0);
     for (int j = 0; j < nth; ++j) ta[j] = at(theta, 0, j, 0);
     for (int k = 0; k < nph; ++k) pa[k] = at(phi, 0, 0, k);
-    for (int i = 0; i < nr && sep; ++i)
-        if (!(ra[i] > 0.0) || !std::isfinite(ra[i]) || (i > 0 && !(ra[i] > ra[i - 1]))) sep = false;
-    for (int j = 0; j < nth && sep; ++j)
-        if (!std::isfinite(ta[j]) || (j > 0 && !(ta[j] > ta[j - 1]))) sep = false;
-    for (int k = 0; k < nph && sep; ++k)
-        if (!(pa[k] >= 0.0) || !(pa[k] < two_pi) || (k > 0 && !(pa[k] > pa[k - 1]))) sep = false;
-    if ((size_t)2 * (nr + nth + nph) * sizeof(double) > kMaxTableBytes) sep = false;
-    out.separable = sep;
     out.tables.clear();
+    out.axes.clear();
+    if (sep) sep = build_search_tables(ra, ta, pa, out.tables);
+    out.separable = sep;
     if (sep) {
-        // within_* thresholds, scattering.hpp:99-149: 0.500001 * neighbour
-        // span; -1 marks a single-entry axis (test distance <= 1e-10 instead).
-        out.tables.assign(2 * (size_t)(nr + nth + nph), 0.0);
-        double* rt = out.tables.data();
-        double* rth = rt + nr;
-        double* tt = rt + 2 * nr;
-        double* tth = tt + nth;
-        double* pt = tt + 2 * nth;
-        double* pth = pt + nph;
-        for (int i = 0; i < nr; ++i) {
-            rt[i] = ra[i];
-            double span = 0.0;
-            if (i > 0 && ra[i - 1] > 0.0) span = std::fmax(span, std::fabs(std::log(ra[i] / ra[i - 1])));
-            if (i + 1 < nr && ra[i + 1] > 0.0)
-                span = std::fmax(span, std::fabs(std::log(ra[i] / ra[i + 1])));
-            rth[i] = nr == 1 ? -1.0 : 0.500001 * span;
-        }
-        for (int j = 0; j < nth; ++j) {
-            tt[j] = ta[j];
-            double span = 0.0;
-            if (j > 0) span = std::fmax(span, std::fabs(ta[j] - ta[j - 1]));
-            if (j + 1 < nth) span = std::fmax(span, std::fabs(ta[j] - ta[j + 1]));
-            tth[j] = nth == 1 ? -1.0 : 0.500001 * span;
-        }
-        auto per = [&](double a, double b) {
-            const double d = std::fmod(std::fabs(a - b), two_pi);
-            return std::fmin(d, two_pi - d);
-        };
-        for (int k = 0; k < nph; ++k) {
-            pt[k] = pa[k];
-            double span = 0.0;
-            if (k > 0) span = std::fmax(span, per(pa[k], pa[k - 1]));
-            if (k + 1 < nph) span = std::fmax(span, per(pa[k], pa[k + 1]));
-            pth[k] = nph == 1 ? -1.0 : 0.500001 * span;
-        }
+        out.axes = ra;
+        out.axes.insert(out.axes.end(), ta.begin(), ta.end());
+        out.axes.insert(out.axes.end(), pa.begin(), pa.end());
     }
     // Cell records: scaled density and temperature, and the Cartesian
     // Kerr-Schild four-velocity / magnetic vector of each cell
@@ -201,7 +237,8 @@ inline void prepare_grid(const grt_params& p, int nr, int nth, int nph, const do
 
 // GridView over host (or device) copies of HostGrid's arrays.
 inline GridView grid_view(int nr, int nth, int nph, bool available, bool separable,
-                          const double* tables, const double* coords3, const double* cells)
+                          const double* tables, const double* axes, const double* coords3,
+                          const double* cells)
 {
     GridView g{};
     g.nr = nr;
@@ -210,12 +247,8 @@ inline GridView grid_view(int nr, int nth, int nph, bool available, bool separab
     g.available = available ? 1 : 0;
     g.separable = separable ? 1 : 0;
     if (available && separable) {
-        g.r_axis = tables;
-        g.r_thr = tables + nr;
-        g.th_axis = tables + 2 * nr;
-        g.th_thr = tables + 2 * nr + nth;
-        g.ph_axis = tables + 2 * nr + 2 * nth;
-        g.ph_thr = tables + 2 * nr + 2 * nth + nph;
+        g.tab = tables;
+        g.axes = axes;
     }
     if (available && !separable) {
         const size_t G = (size_t)nr * nth * nph;

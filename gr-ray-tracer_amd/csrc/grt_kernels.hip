@@ -85,32 +85,16 @@ __device__ __forceinline__ void store_photon(const PhotonPtrs& p, uint32_t idx, 
 #define GRT_TRACE_WAVES_PER_EU 2
 #endif
 
-template <int kIntegrator>
+template <int kIntegrator, int kSep>
 __global__ __launch_bounds__(256, GRT_TRACE_WAVES_PER_EU) void trace_fp64(TraceLaunch a)
 {
     extern __shared__ double lds[];
     GridView G = a.grid;
-    if (a.use_lds) {
-        // Stage the separable search tables (axes + within thresholds).
-        const int n = a.lds_doubles;
-        const int nr = G.nr, nth = G.nth;
-        for (int t = threadIdx.x; t < n; t += blockDim.x) {
-            double v;
-            if (t < nr) v = G.r_axis[t];
-            else if (t < 2 * nr) v = G.r_thr[t - nr];
-            else if (t < 2 * nr + nth) v = G.th_axis[t - 2 * nr];
-            else if (t < 2 * nr + 2 * nth) v = G.th_thr[t - 2 * nr - nth];
-            else if (t < 2 * nr + 2 * nth + G.nph) v = G.ph_axis[t - 2 * nr - 2 * nth];
-            else v = G.ph_thr[t - 2 * nr - 2 * nth - G.nph];
-            lds[t] = v;
-        }
+    if (kSep && a.use_lds) {
+        // Stage the separable search tables in LDS.
+        for (int t = threadIdx.x; t < a.lds_doubles; t += blockDim.x) lds[t] = G.tab[t];
         __syncthreads();
-        G.r_axis = lds;
-        G.r_thr = lds + nr;
-        G.th_axis = lds + 2 * nr;
-        G.th_thr = lds + 2 * nr + nth;
-        G.ph_axis = lds + 2 * nr + 2 * nth;
-        G.ph_thr = lds + 2 * nr + 2 * nth + G.nph;
+        G.tab = lds;
     }
     const StepParams& P = a.consts->P;
     const PhotonPtrs& PH = a.consts->ph;
@@ -132,7 +116,7 @@ __global__ __launch_bounds__(256, GRT_TRACE_WAVES_PER_EU) void trace_fp64(TraceL
 
     while (have) {
         double au = 0.0, av = 0.0;
-        const bool call_done = lane_iterate<kIntegrator>(L, G, P, rng, n, au, av);
+        const bool call_done = lane_iterate<kIntegrator, kSep>(L, G, P, rng, n, au, av);
         if (call_done) {
             ++n.calls;
             ++calls_done;
@@ -286,10 +270,18 @@ hipError_t launch_init_image(const PhotonPtrs& ph, uint64_t n, uint64_t id0, con
 hipError_t launch_trace(const TraceLaunch& a, int blocks, int block, size_t lds_bytes,
                         hipStream_t stream)
 {
-    if (a.integrator == GRT_RK4)
-        hipLaunchKernelGGL(trace_fp64<GRT_RK4>, dim3(blocks), dim3(block), lds_bytes, stream, a);
-    else
-        hipLaunchKernelGGL(trace_fp64<GRT_RK45>, dim3(blocks), dim3(block), lds_bytes, stream, a);
+    const bool sep = a.grid.separable || !a.grid.available;
+    if (a.integrator == GRT_RK4) {
+        if (sep)
+            hipLaunchKernelGGL((trace_fp64<GRT_RK4, 1>), dim3(blocks), dim3(block), lds_bytes, stream, a);
+        else
+            hipLaunchKernelGGL((trace_fp64<GRT_RK4, 0>), dim3(blocks), dim3(block), lds_bytes, stream, a);
+    } else {
+        if (sep)
+            hipLaunchKernelGGL((trace_fp64<GRT_RK45, 1>), dim3(blocks), dim3(block), lds_bytes, stream, a);
+        else
+            hipLaunchKernelGGL((trace_fp64<GRT_RK45, 0>), dim3(blocks), dim3(block), lds_bytes, stream, a);
+    }
     return hipGetLastError();
 }
 
@@ -321,7 +313,7 @@ hipError_t trace_kernel_attributes(int* vgprs, int* sgprs, int* lds_static, int*
 {
     hipFuncAttributes attr;
     const hipError_t e = hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(
-                                                         &trace_fp64<GRT_RK45>));
+                                                         &trace_fp64<GRT_RK45, 1>));
     if (e != hipSuccess) return e;
     *vgprs = 0;
     *sgprs = 0;

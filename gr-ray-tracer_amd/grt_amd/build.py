@@ -29,7 +29,9 @@ def hipcc():
 
 
 def compile_flags(resource_usage=False):
-    flags = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    # -ffp-contract=off: the FP64 path rounds every +, -, * exactly as the
+    # reference's expressions do (no fused multiply-add), see grt_device.hpp.
+    flags = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function",
              "-I" + INCLUDE, "-I" + CSRC]

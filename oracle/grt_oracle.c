@@ -276,6 +276,28 @@ void orc_geodesic_observables(const double s[8], double a, double m, double out[
 
 #define ORC_NMAX 16
 
+/* pow(x, 0.25) correctly rounded (integrate_cart_ks.hpp:217 calls the
+ * platform pow, whose last bit varies between libraries; glibc's is not
+ * correctly rounded for ~0.1% of inputs).  sqrt(sqrt(x)) is within an ulp;
+ * of it and its neighbours keep the one whose fourth power, evaluated with
+ * exact fma products, is closest to x. */
+static double quarter_root(double x)
+{
+    const double y = sqrt(sqrt(x));
+    double best = y, bestd = INFINITY;
+    for (int d = -1; d <= 1; ++d) {
+        const double c = d < 0 ? nextafter(y, 0.0) : (d > 0 ? nextafter(y, INFINITY) : y);
+        const double s = c * c;
+        const double e1 = fma(c, c, -s);
+        const double q = s * s;
+        const double e2 = fma(s, s, -q);
+        const double r = fabs((q - x) + (e2 + 2.0 * s * e1));
+        if (r < bestd) { bestd = r; best = c; }
+    }
+    return best;
+}
+
+
 /* rk_integrators.hpp:12-22 */
 void orc_rk4_generic(int n, double *state, double dt, orc_rhs_fn f, void *user)
 {
@@ -328,7 +350,7 @@ int orc_rk45_generic(int n, double *state, double *dt, const double *k1, orc_rhs
         for (int i = 0; i < n; ++i) state[i] = x5[i];
         return 1;
     }
-    double scale = safety * pow(1.0 / err, 0.25);
+    double scale = safety * quarter_root(1.0 / err);
     scale = fmin(max_s, fmax(min_s, scale));
     *dt = h * scale;
     if (err <= 1.0) {

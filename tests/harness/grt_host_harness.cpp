@@ -4,6 +4,7 @@
 // the CPU so their results can be compared with the oracle without a GPU.
 // Photons run one after another to termination or max_steps calls, which is
 // the reference loop's outcome for termination_percent = 1.
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -38,7 +39,7 @@ void prepare(const grt_params* p, int nr, int nth, int nph, const double* r, con
         }
     }
     out.view = grid_view(nr, nth, nph, avail, avail && out.hg.separable, out.hg.tables.data(),
-                         out.coords.data(), out.hg.cells.data());
+                         out.hg.axes.data(), out.coords.data(), out.hg.cells.data());
 }
 
 }  // namespace
@@ -54,8 +55,11 @@ int harness_locate(const grt_params* p, int nr, int nth, int nph, const double* 
     prepare(p, nr, nth, nph, r, th, ph, rho, tg, vel, mag, force_general, pr);
     for (int64_t q = 0; q < nq; ++q) {
         int i = -1, j = -1, k = -1;
+        // the kernel derives (cos theta, X, Y) from the position; here they
+        // come from the query's (r, theta, phi) with a = 0: X, Y = cos, sin phi
         const bool ok = pr.view.separable
-                            ? locate_separable(pr.view, radius[q], polar[q], az[q], i, j, k)
+                            ? locate_separable(pr.view, radius[q], std::cos(polar[q]),
+                                               std::cos(az[q]), std::sin(az[q]), i, j, k)
                             : locate_general(pr.view, radius[q], polar[q], az[q], i, j, k);
         out[4 * q + 0] = ok ? 1 : 0;
         out[4 * q + 1] = i;
@@ -97,11 +101,15 @@ int harness_trace(const grt_params* p, int nr, int nth, int nph, const double* r
         double au = 0.0, av = 0.0;
         while (calls < max_steps && !f.term) {
             au = av = 0.0;
-            if (sp.integrator == GRT_RK4) {
-                if (lane_iterate<GRT_RK4>(L, pr.view, sp, rng, cnt, au, av)) { ++calls; ++cnt.calls; }
-            } else {
-                if (lane_iterate<GRT_RK45>(L, pr.view, sp, rng, cnt, au, av)) { ++calls; ++cnt.calls; }
-            }
+            bool done;
+            const bool sep = pr.view.separable || !pr.view.available;
+            if (sp.integrator == GRT_RK4)
+                done = sep ? lane_iterate<GRT_RK4, 1>(L, pr.view, sp, rng, cnt, au, av)
+                           : lane_iterate<GRT_RK4, 0>(L, pr.view, sp, rng, cnt, au, av);
+            else
+                done = sep ? lane_iterate<GRT_RK45, 1>(L, pr.view, sp, rng, cnt, au, av)
+                           : lane_iterate<GRT_RK45, 0>(L, pr.view, sp, rng, cnt, au, av);
+            if (done) { ++calls; ++cnt.calls; }
         }
         P->x0[i] = f.s[0]; P->x1[i] = f.s[1]; P->x2[i] = f.s[2]; P->x3[i] = f.s[3];
         P->k0[i] = f.s[4]; P->k1[i] = f.s[5]; P->k2[i] = f.s[6]; P->k3[i] = f.s[7];
