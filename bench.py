@@ -40,6 +40,12 @@ F_REJ = 1.73e3
 F_FLUID = 1.1e3
 PEAK_FP64_TFLOPS = 78.6     # MI355X vector FP64, published
 
+# HBM bytes per trace_fp64 launch of the default workload, from two
+# rocprofv3 --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, the factors a
+# plain 512 MiB device copy reads on the same counters); the profile summary
+# profiles/trace_fp64_rk45_fm_torus_256x128x128_1024px.md holds the commands.
+TRAFFIC_BYTES_DEFAULT = 953.6e6
+
 
 def workload(args):
     from grt_amd.fm_torus import fishbone_moncrief
@@ -60,11 +66,15 @@ def workload(args):
 
 
 def cpu_baseline(fields, kw, su, sv, sample, threads, max_steps):
-    """The oracle (FP64 C/OpenMP restatement, reference O(n) cell search) on a
-    strided subset of the same rays."""
+    """The oracle (FP64 C/OpenMP restatement, reference O(n) cell search) on
+    the same rays: all of them when sample is 0, else an evenly strided
+    subset."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import grt_oracle as O
-    idx = np.linspace(0, len(su) - 1, sample).astype(np.int64)
+    if sample <= 0 or sample >= len(su):
+        idx = np.arange(len(su), dtype=np.int64)
+    else:
+        idx = np.linspace(0, len(su) - 1, sample).astype(np.int64)
     po = O.make_params(**kw)
     ph = O.init_image(po, su[idx], sv[idx], 0.1)
     G = O.GridHolder(fields, kw["a_bh"], kw["m_bh"])
@@ -84,10 +94,14 @@ def main():
     ap.add_argument("--nth", type=int, default=128)
     ap.add_argument("--nph", type=int, default=128)
     ap.add_argument("--max-steps", type=int, default=10000)
-    ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="rays traced by the CPU baseline (0: the whole image)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy of MIB MiB and exit (calibrates "
+                         "the FETCH_SIZE / WRITE_SIZE counters against known traffic)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -102,6 +116,16 @@ def main():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
     dev = torch.device("cuda:0")
+
+    if args.copy_probe > 0:
+        nbytes = args.copy_probe << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": nbytes, "checksum": float(dst[-1].item())}))
+        return
 
     def log(*a):
         if not args.quiet:
@@ -144,6 +168,7 @@ def main():
     img = ctx.image_products(batches[-1], args.image, args.image)
     arrived = img["arrived_count"]
 
+    default_workload = (args.image, args.nr, args.nth, args.nph) == (1024, 256, 128, 128)
     steps_per_s = totals["accepted_steps"] / wall
     flops = (totals["accepted_steps"] * F_ACC + totals["rejected_attempts"] * F_REJ +
              totals["fluid_samples"] * F_FLUID)
@@ -183,7 +208,7 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": None,
+            "traffic": TRAFFIC_BYTES_DEFAULT if default_workload else None,
             "kernel": "trace_fp64<RK45>",
             "avg_launch_ms": avg_launch_ms,
             "flops_per_launch": flops / max(1, totals["launches"]),
@@ -194,13 +219,16 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         idx, ph_o, st_o, dt = cpu_baseline(fields, kw, su, sv, args.cpu_sample, threads,
                                            args.max_steps)
+        full = len(idx) == n
+        what = (f"all {n} rays of one render" if full else
+                f"{len(idx)} of the {n} rays (evenly strided)")
         result["cpu_baseline"] = {
             "value": st_o["accepted_steps"] / dt,
             "unit": "photon RK45 steps/s",
             "cores": threads,
             "kind": "port",
-            "sample": f"{args.cpu_sample} of the {n} rays (evenly strided), same grid and "
-                      f"settings, {dt:.1f} s wall on {threads} threads",
+            "sample": f"{what}, same grid and settings, {st_o['accepted_steps']} steps in "
+                      f"{dt:.1f} s wall on {threads} threads",
         }
         # parity of the sampled rays against the GPU render of the same rays
         X = np.stack([last[k][idx] for k in ("x0", "x1", "x2", "x3")])
@@ -212,6 +240,16 @@ def main():
             "x_rel_err_max": float(ex.max()),
             "x_within_1e-9": float(np.mean(ex <= 1e-9)),
         }
+        if full:
+            import grt_oracle as O
+            io = O.image_products(ph_o, args.image, args.image, 128, 1e8, 1e22,
+                                  kw["camera_width"], kw["camera_height"])
+            Ig = img["image"].cpu().numpy()
+            Io = io["image"]
+            scale = np.maximum(np.abs(Io), 1e-300)
+            rel = np.where(Io != 0, np.abs(Ig - Io) / scale, np.abs(Ig))
+            result["parity_sample"]["pixel_stokes_rel_err_max"] = float(rel.max())
+            result["parity_sample"]["pixels_within_1e-8"] = float(np.mean(rel <= 1e-8))
     print(json.dumps(result), flush=True)
 
 

@@ -163,15 +163,43 @@ __global__ __launch_bounds__(256, GRT_TRACE_WAVES_PER_EU) void trace_fp64(TraceL
     }
 }
 
+// Live list of the photons not yet terminated.  Each thread tests
+// kLiveItems consecutive entries; the block scans its counts and takes its
+// slice of the list with one atomic, so entries stay ascending within a block.
+constexpr int kLiveItems = 8;
+
 __global__ __launch_bounds__(256) void build_live(const uint8_t* __restrict__ term, uint64_t n,
                                                   uint32_t* __restrict__ list,
                                                   uint32_t* __restrict__ count)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && !term[i]) {
-        const uint32_t o = wave_claim(count);
-        list[o] = (uint32_t)i;
+    __shared__ uint32_t wave_total[4];
+    __shared__ uint32_t block_base;
+    const uint64_t first = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kLiveItems;
+    uint32_t live = 0;
+#pragma unroll
+    for (int q = 0; q < kLiveItems; ++q)
+        if (first + q < n && !term[first + q]) live |= 1u << q;
+    const uint32_t c = (uint32_t)__popc(live);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t incl = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
     }
+    if (lane == 63) wave_total[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += wave_total[w];
+        block_base = tot ? atomicAdd(count, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t o = block_base + incl - c;
+    for (int w = 0; w < wave; ++w) o += wave_total[w];
+#pragma unroll
+    for (int q = 0; q < kLiveItems; ++q)
+        if (live & (1u << q)) list[o++] = (uint32_t)(first + q);
 }
 
 // observation_bin, observation.hpp:22-29
@@ -183,6 +211,11 @@ __device__ __forceinline__ int64_t obs_bin(double v, double lo, double hi, int64
     return (int64_t)(sc * (double)(bins - 1));
 }
 
+// Spectra of up to kLdsBins bins are summed per block in LDS first: every
+// photon of a one-frequency run falls in the same bin, and direct atomics on
+// that one address would serialise the whole launch.
+constexpr int kLdsBins = 4096;
+
 // build_observation_products, observation.hpp:31-84 (sums by atomics).
 __global__ __launch_bounds__(256) void image_products(PhotonPtrs ph, uint64_t n, int64_t nx,
                                                       int64_t ny, int64_t bins, double smin,
@@ -190,6 +223,12 @@ __global__ __launch_bounds__(256) void image_products(PhotonPtrs ph, uint64_t n,
                                                       double w, double h, double* image,
                                                       double* spec, unsigned long long* arrived)
 {
+    __shared__ double hist[kLdsBins];
+    const bool lds = bins <= kLdsBins;
+    if (lds) {
+        for (int b = threadIdx.x; b < bins; b += blockDim.x) hist[b] = 0.0;
+        __syncthreads();
+    }
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool hit = false;
     if (i < n && ph.phase[i] == kArrived) {
@@ -206,12 +245,18 @@ __global__ __launch_bounds__(256) void image_products(PhotonPtrs ph, uint64_t n,
         const double nu = ph.freq[i];
         if (nu >= smin && nu <= smax) {
             const int64_t b = obs_bin(log10(nu), lmin, lmax, bins);
-            unsafeAtomicAdd(&spec[b], I);
+            if (lds) atomicAdd(&hist[b], I);
+            else unsafeAtomicAdd(&spec[b], I);
         }
     }
     const uint64_t mask = __ballot(hit);
     if (mask && (threadIdx.x & 63) == (uint32_t)(__ffsll((long long)mask) - 1))
         atomicAdd(arrived, (unsigned long long)__popcll(mask));
+    if (lds) {
+        __syncthreads();
+        for (int b = threadIdx.x; b < bins; b += blockDim.x)
+            if (hist[b] != 0.0) unsafeAtomicAdd(&spec[b], hist[b]);
+    }
 }
 
 // initialize_photons_image, initialize_photons.hpp:82-170.  Screen
@@ -290,7 +335,8 @@ hipError_t launch_build_live(const uint8_t* term, uint64_t n, uint32_t* list, ui
 {
     if (n == 0) return hipSuccess;
     const int block = 256;
-    const uint64_t blocks = (n + block - 1) / block;
+    const uint64_t per_block = (uint64_t)block * kLiveItems;
+    const uint64_t blocks = (n + per_block - 1) / per_block;
     hipLaunchKernelGGL(build_live, dim3((unsigned)blocks), dim3(block), 0, stream, term, n, list,
                        count);
     return hipGetLastError();
