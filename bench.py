@@ -65,6 +65,88 @@ def workload(args):
     return fields, kw, U.ravel().copy(), V.ravel().copy()
 
 
+def timed_renders(ctx, proto, steps, warmup, max_steps):
+    """Trace warmup + steps fresh copies of `proto` (all resident before the
+    clock starts); returns (wall seconds of the timed renders, summed stats,
+    the last batch)."""
+    import torch
+    batches = [{k: v.clone() for k, v in proto.items()} for _ in range(warmup + steps)]
+    for w in range(warmup):
+        ctx.trace(batches[w], max_steps)
+    totals = dict(accepted_steps=0, rejected_attempts=0, functor_calls=0, fluid_samples=0,
+                  failed_steps=0, global_steps=0, launches=0, kernel_ms=0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        st = ctx.trace(batches[warmup + k], max_steps)
+        for key in totals:
+            totals[key] += st[key]
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    last = batches[-1]
+    del batches
+    return wall, totals, last
+
+
+def precision_leg(args, log):
+    """The precision form: a 2048^2 image of the a = 0.94 torus (r_in = 6,
+    r_pmax = 12, no density floor, 256 x 128 x 128) at 17 deg, RK45
+    rtol = atol = 1e-7, rendered by the FP64 kernel and by the mixed
+    FP32/FP64 kernel from the same initial rays."""
+    import torch
+    import grt_amd
+    from grt_amd.fm_torus import fishbone_moncrief
+    from grt_amd.image_metrics import pixel_errors, ring_diameter
+    dev = torch.device("cuda:0")
+    spin = 0.94
+    fields = fishbone_moncrief(nr=args.nr, ntheta=args.nth, nphi=args.nph, r_min=1.5,
+                               r_max=100.0, mass=1.0, spin=spin, r_in=6.0,
+                               r_pressure_max=12.0, rho_floor=0.0)
+    kw = dict(a_bh=spin, m_bh=1.0, r_term_min=1.0 + np.sqrt(1.0 - spin * spin),
+              r_term_max=1500.0, atol=1e-7, rtol=1e-7, integrator=1, mode=0,
+              absorption_enabled=1, emission_enabled=1, absorption_coefficient=1.0,
+              emission_coefficient=1.0, superphoton_count_normalization=1e9,
+              camera_distance=1000.0, camera_theta=np.radians(17.0), camera_phi=0.0,
+              camera_width=50.0, camera_height=50.0)
+    n = args.precision_image
+    u = -25.0 + (np.arange(n) + 0.5) * 50.0 / n
+    U, V = np.meshgrid(u, u, indexing="xy")
+    su = torch.from_numpy(U.ravel().copy()).to(dev)
+    sv = torch.from_numpy(V.ravel().copy()).to(dev)
+    out = {}
+    for name, prec in (("fp64", grt_amd.PRECISION_FP64), ("mixed", grt_amd.PRECISION_MIXED)):
+        ctx = grt_amd.Context(grt_amd.default_params(precision=prec, **kw))
+        ctx.set_grid(fields)
+        proto = grt_amd.empty_photons(n * n, device=dev)
+        ctx.init_image(proto, 0.1, su, sv)
+        wall, tot, last = timed_renders(ctx, proto, args.precision_steps, 1, args.max_steps)
+        img = ctx.image_products(last, n, n)
+        out[name] = dict(wall=wall, totals=tot, image=img["image"][0].cpu().numpy())
+        log(f"precision form {name}: {args.precision_steps} renders in {wall * 1e3:.1f} ms, "
+            f"{tot['accepted_steps'] / wall:.3e} steps/s")
+        del proto, last, ctx
+        torch.cuda.empty_cache()
+    f, m = out["fp64"], out["mixed"]
+    rate_f = f["totals"]["accepted_steps"] / f["wall"]
+    rate_m = m["totals"]["accepted_steps"] / m["wall"]
+    err = pixel_errors(m["image"], f["image"])
+    d_f, d_m = ring_diameter(f["image"], 50.0), ring_diameter(m["image"], 50.0)
+    return {
+        "workload": f"{n}x{n} rays, FM torus a=0.94 without density floor on "
+                    f"{args.nr}x{args.nth}x{args.nph}, inclination 17 deg, RK45 rtol 1e-7",
+        "renders": args.precision_steps,
+        "fp64_steps_per_s": rate_f,
+        "mixed_steps_per_s": rate_m,
+        "speedup": rate_m / rate_f,
+        "pixels_per_s_fp64": n * n * args.precision_steps / f["wall"],
+        "pixels_per_s_mixed": n * n * args.precision_steps / m["wall"],
+        "accepted_steps_per_render": {"fp64": f["totals"]["accepted_steps"] / args.precision_steps,
+                                      "mixed": m["totals"]["accepted_steps"] / args.precision_steps},
+        "stokes_I_vs_fp64": err,
+        "ring_diameter_M": {"fp64": d_f, "mixed": d_m, "abs_err": abs(d_m - d_f)},
+    }
+
+
 def cpu_baseline(fields, kw, su, sv, sample, threads, max_steps):
     """The oracle (FP64 C/OpenMP restatement, reference O(n) cell search) on
     the same rays: all of them when sample is 0, else an evenly strided
@@ -98,6 +180,10 @@ def main():
                     help="rays traced by the CPU baseline (0: the whole image)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the FP64-vs-mixed comparison on the 2048^2 a=0.94 image")
+    ap.add_argument("--precision-image", type=int, default=2048)
+    ap.add_argument("--precision-steps", type=int, default=3)
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB and exit (calibrates "
@@ -214,6 +300,11 @@ def main():
             "flops_per_launch": flops / max(1, totals["launches"]),
         },
     }
+
+    if not args.no_precision:
+        del batches
+        torch.cuda.empty_cache()
+        result["mixed_precision"] = precision_leg(args, log)
 
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

@@ -439,8 +439,6 @@ int grt_trace_device(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max
     if (max_steps <= 0) return fail(c, GRT_E_INVALID, "integration.max_steps must be positive");
     if (!(termination_percent > 0.0) || termination_percent > 1.0)
         return fail(c, GRT_E_INVALID, "integration.termination_percent must be in (0, 1]");
-    if (c->params.precision != GRT_PRECISION_FP64)
-        return fail(c, GRT_E_INVALID, "the mixed-precision path is not available in this build");
     hipStream_t stream = static_cast<hipStream_t>(stream_v);
     GRT_HIP(c, hipSetDevice(c->device));
     int rc = ensure_scratch(c, n);
@@ -477,6 +475,7 @@ int grt_trace_device(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max
     const int lds_doubles = (c->has_grid && c->separable) ? (int)c->table_doubles : 0;
     L.use_lds = lds_doubles > 0 ? 1 : 0;
     L.lds_doubles = lds_doubles;
+    L.mixed = c->params.precision == GRT_PRECISION_MIXED ? 1 : 0;
     const size_t lds_bytes = (size_t)lds_doubles * sizeof(double);
     const int block = c->tune.block;
     const int persistent_blocks = std::max(1, c->num_cus * c->tune.blocks_per_cu);

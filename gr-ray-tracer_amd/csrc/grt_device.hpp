@@ -16,6 +16,8 @@
 
 #include "grt_extension_points.h"
 
+#include <type_traits>
+
 #define GRT_HD __host__ __device__ __forceinline__
 
 namespace grt {
@@ -133,8 +135,9 @@ GRT_HD void ks_rhs(const double s[8], double a, double m, double out[8])
     const double D = r3 * r + az * az;
     const double D2 = D * D;
 #if GRT_RHS_RECIPROCALS
-    // Four reciprocals replace the 21 divisions (each quotient then differs
-    // from the reference's by at most an ulp or two).
+    // (mixed-precision form) Four reciprocals replace the 21 divisions and
+    // each quotient is a plain product with one of them, within an ulp or
+    // two of the reference's quotient.
     const double iS = 1.0 / S, itr = 1.0 / tr, iB = 1.0 / B, iD = 1.0 / D;
     const double ir = 2.0 * itr;
     const double iD2 = iD * iD;
@@ -197,6 +200,14 @@ GRT_HD void ks_rhs(const double s[8], double a, double m, double out[8])
     out[7] = dH_z * lp2 + hl * dlp_z;
 }
 
+// Right-hand-side policy of the integrators: the FP64 evaluation above.
+struct RhsExact {
+    GRT_HD static void eval(const double s[8], const StepParams& P, double out[8])
+    {
+        ks_rhs(s, P.a, P.m, out);
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Integrators, integrate_cart_ks.hpp:143-227
 // ---------------------------------------------------------------------------
@@ -243,17 +254,18 @@ GRT_HD double quarter_root(double x)
 #define GRT_FOR_STATE(i) _Pragma("unroll") for (int i = 0; i < 8; ++i) if (i != 4)
 
 // Classic RK4, integrate_cart_ks.hpp:143-155.
-GRT_HD void rk4_step(double s[8], double h, double a, double m)
+template <class Rhs = RhsExact>
+GRT_HD void rk4_step(double s[8], double h, const StepParams& P)
 {
     double k1[8], k2[8], k3[8], k4[8], t[8];
-    ks_rhs(s, a, m, k1);
+    Rhs::eval(s, P, k1);
     t[4] = s[4];
     GRT_FOR_STATE(i) t[i] = s[i] + 0.5 * h * k1[i];
-    ks_rhs(t, a, m, k2);
+    Rhs::eval(t, P, k2);
     GRT_FOR_STATE(i) t[i] = s[i] + 0.5 * h * k2[i];
-    ks_rhs(t, a, m, k3);
+    Rhs::eval(t, P, k3);
     GRT_FOR_STATE(i) t[i] = s[i] + h * k3[i];
-    ks_rhs(t, a, m, k4);
+    Rhs::eval(t, P, k4);
     const double h6 = h / 6.0;
     GRT_FOR_STATE(i) s[i] += h6 * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]);
 }
@@ -266,20 +278,25 @@ GRT_HD void rk4_step(double s[8], double h, double a, double m)
 // available (P4 = a41 k1 + a42 k2, later P4 + a43 k3, ...), which is exactly
 // the evaluation order of the reference's expressions, so only the partial
 // sums -- not k2..k6 -- stay live in registers.
+//
+// kMixed evaluates the controller's err^(-1/4) in FP32: the proposal only
+// needs a few digits, and the exact form's correctly rounded root costs more
+// than the rest of the error norm.
+template <class Rhs = RhsExact, int kMixed = 0>
 GRT_HD bool rk45_attempt(double s[8], double* h_io, const double k1[8], const StepParams& P)
 {
     const double h = *h_io;
     double k[8], t[8], P4[8], P5[8], P6[8], X4[8], X5[8];
     t[4] = s[4];
     GRT_FOR_STATE(i) t[i] = s[i] + h * A21 * k1[i];
-    ks_rhs(t, P.a, P.m, k);                                   // k2
+    Rhs::eval(t, P, k);                                   // k2
     GRT_FOR_STATE(i) {
         t[i] = s[i] + h * (A31 * k1[i] + A32 * k[i]);
         P4[i] = A41 * k1[i] + A42 * k[i];
         P5[i] = A51 * k1[i] + A52 * k[i];
         P6[i] = A61 * k1[i] + A62 * k[i];
     }
-    ks_rhs(t, P.a, P.m, k);                                   // k3
+    Rhs::eval(t, P, k);                                   // k3
     GRT_FOR_STATE(i) {
         t[i] = s[i] + h * (P4[i] + A43 * k[i]);
         P5[i] = P5[i] + A53 * k[i];
@@ -287,20 +304,20 @@ GRT_HD bool rk45_attempt(double s[8], double* h_io, const double k1[8], const St
         X4[i] = B41 * k1[i] + B43 * k[i];
         X5[i] = B51 * k1[i] + B53 * k[i];
     }
-    ks_rhs(t, P.a, P.m, k);                                   // k4
+    Rhs::eval(t, P, k);                                   // k4
     GRT_FOR_STATE(i) {
         t[i] = s[i] + h * (P5[i] + A54 * k[i]);
         P6[i] = P6[i] + A64 * k[i];
         X4[i] = X4[i] + B44 * k[i];
         X5[i] = X5[i] + B54 * k[i];
     }
-    ks_rhs(t, P.a, P.m, k);                                   // k5
+    Rhs::eval(t, P, k);                                   // k5
     GRT_FOR_STATE(i) {
         t[i] = s[i] + h * (P6[i] + A65 * k[i]);
         X4[i] = X4[i] + B45 * k[i];
         X5[i] = X5[i] + B55 * k[i];
     }
-    ks_rhs(t, P.a, P.m, k);                                   // k6
+    Rhs::eval(t, P, k);                                   // k6
 
     double x5[8];
     double err = 0.0;
@@ -322,7 +339,10 @@ GRT_HD bool rk45_attempt(double s[8], double* h_io, const double k1[8], const St
     if (err == 0.0) {
         scale = P.smax;
     } else {
-        scale = P.safety * quarter_root(1.0 / err);
+        if (kMixed)
+            scale = P.safety * (double)(1.0f / sqrtf(sqrtf((float)err)));
+        else
+            scale = P.safety * quarter_root(1.0 / err);
         scale = fmin(P.smax, fmax(P.smin, scale));
     }
     *h_io = h * scale;
@@ -599,39 +619,51 @@ GRT_HD KSMetric ks_metric_at(const double s[8], double a, double m)
 
 // metric_inner_product (fluid_frame.hpp:6-14) through the factored metric:
 // eta(u, v) + twoH (l.u)(l.v).  Equal to the 16-term sum up to rounding.
+// T is the arithmetic type (double; float in the mixed-precision form).
+template <class T, class U, class V>
+GRT_HD T inner(const KSMetric& g, const U u[4], const V v[4])
+{
+    const T l1 = (T)g.l1, l2 = (T)g.l2, l3 = (T)g.l3;
+    const T u0 = (T)u[0], u1 = (T)u[1], u2 = (T)u[2], u3 = (T)u[3];
+    const T v0 = (T)v[0], v1 = (T)v[1], v2 = (T)v[2], v3 = (T)v[3];
+    const T lu = u0 + l1 * u1 + l2 * u2 + l3 * u3;
+    const T lv = v0 + l1 * v1 + l2 * v2 + l3 * v3;
+    return (-u0 * v0 + u1 * v1 + u2 * v2 + u3 * v3) + (T)g.twoH * lu * lv;
+}
+
 GRT_HD double inner(const KSMetric& g, const double u[4], const double v[4])
 {
-    const double lu = u[0] + g.l1 * u[1] + g.l2 * u[2] + g.l3 * u[3];
-    const double lv = v[0] + g.l1 * v[1] + g.l2 * v[2] + g.l3 * v[3];
-    return (-u[0] * v[0] + u[1] * v[1] + u[2] * v[2] + u[3] * v[3]) + g.twoH * lu * lv;
+    return inner<double>(g, u, v);
 }
 
 // build_fluid_tetrad, fluid_frame.hpp:52-80: e[0] = u / |u|, then
-// Gram-Schmidt of the coordinate x, y, z axes under g.
-GRT_HD bool fluid_tetrad(const KSMetric& g, const double u[4], double e[16])
+// Gram-Schmidt of the coordinate x, y, z axes under g, in arithmetic T.
+template <class T>
+GRT_HD bool fluid_tetrad(const KSMetric& g, const double u_in[4], T e[16])
 {
-    const double vn = inner(g, u, u);
-    if (!(vn < 0.0) || !is_finite(vn)) return false;
-    const double inv = 1.0 / sqrt(-vn);
-    const double sgn = u[0] >= 0.0 ? 1.0 : -1.0;
+    const T u[4] = {(T)u_in[0], (T)u_in[1], (T)u_in[2], (T)u_in[3]};
+    const T vn = inner<T>(g, u, u);
+    if (!(vn < (T)0) || !is_finite((double)vn)) return false;
+    const T inv = (T)1 / sqrt(-vn);
+    const T sgn = u[0] >= (T)0 ? (T)1 : (T)-1;
 #pragma unroll
     for (int mu = 0; mu < 4; ++mu) e[mu] = sgn * u[mu] * inv;
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
-        double c[4] = {0.0, 0.0, 0.0, 0.0};
-        c[ax + 1] = 1.0;
-        const double tp = inner(g, e, c);
+        T c[4] = {(T)0, (T)0, (T)0, (T)0};
+        c[ax + 1] = (T)1;
+        const T tp = inner<T>(g, e, c);
 #pragma unroll
         for (int mu = 0; mu < 4; ++mu) c[mu] += tp * e[mu];
 #pragma unroll
         for (int pv = 0; pv < ax; ++pv) {
-            const double pr = inner(g, e + 4 * (pv + 1), c);
+            const T pr = inner<T>(g, e + 4 * (pv + 1), c);
 #pragma unroll
             for (int mu = 0; mu < 4; ++mu) c[mu] -= pr * e[4 * (pv + 1) + mu];
         }
-        const double nn = inner(g, c, c);
-        if (!(nn > 1.0e-20) || !is_finite(nn)) return false;
-        const double is = 1.0 / sqrt(nn);
+        const T nn = inner<T>(g, c, c);
+        if (!((double)nn > 1.0e-20) || !is_finite((double)nn)) return false;
+        const T is = (T)1 / sqrt(nn);
 #pragma unroll
         for (int mu = 0; mu < 4; ++mu) e[4 * (ax + 1) + mu] = c[mu] * is;
     }
@@ -641,9 +673,9 @@ GRT_HD bool fluid_tetrad(const KSMetric& g, const double u[4], double e[16])
 // sample_scattering_fluid, scattering.hpp:185-251.  `g` receives the metric at
 // the photon position (reused by the scattering update).
 // kSep selects the separable fast search at compile time.
-template <int kSep>
+template <int kSep, class T>
 GRT_HD bool sample_fluid(const double s[8], const GridView& G, const StepParams& P,
-                         FluidCellCGS& fl, double e[16], KSMetric& g)
+                         FluidCellCGS& fl, T e[16], KSMetric& g)
 {
     if (!G.available) return false;
     const double radius = ks_radius(s[1], s[2], s[3], P.a);
@@ -674,12 +706,12 @@ GRT_HD bool sample_fluid(const double s[8], const GridView& G, const StepParams&
 #pragma unroll
     for (int q = 0; q < 4; ++q) fl.coordinate_four_velocity_cm_s[q] = c[2 + q];
     g = ks_metric_at(s, P.a, P.m);
-    if (!fluid_tetrad(g, fl.coordinate_four_velocity_cm_s, e)) return false;
+    if (!fluid_tetrad<T>(g, fl.coordinate_four_velocity_cm_s, e)) return false;
     const double b[4] = {c[6], c[7], c[8], c[9]};
-    fl.magnetic_gauss[0] = -inner(g, e, b);
-    fl.magnetic_gauss[1] = inner(g, e + 4, b);
-    fl.magnetic_gauss[2] = inner(g, e + 8, b);
-    fl.magnetic_gauss[3] = inner(g, e + 12, b);
+    fl.magnetic_gauss[0] = -(double)inner<T>(g, e, b);
+    fl.magnetic_gauss[1] = (double)inner<T>(g, e + 4, b);
+    fl.magnetic_gauss[2] = (double)inner<T>(g, e + 8, b);
+    fl.magnetic_gauss[3] = (double)inner<T>(g, e + 12, b);
     return true;
 }
 
@@ -771,8 +803,9 @@ constexpr double kProtonMassCgs = 1.67262192369e-24;    // scattering.hpp:256
 constexpr double kSigmaThomsonCgs = 6.6524587158e-25;   // scattering.hpp:257
 
 // maybe_scatter_photon, scattering.hpp:259-313 (g = metric at the photon).
+template <class T>
 GRT_HD bool maybe_scatter(double s[8], double stokes[4], const StepParams& P, double dl_abs,
-                          const FluidCellCGS& fl, const double e[16], const KSMetric& g,
+                          const FluidCellCGS& fl, const T e[16], const KSMetric& g,
                           PhotonRng& rng)
 {
     const double path = dl_abs * P.length_scale;
@@ -782,7 +815,7 @@ GRT_HD bool maybe_scatter(double s[8], double stokes[4], const StepParams& P, do
     if (rng.drand() >= prob) return false;
     double eloc = 0.0;
 #pragma unroll
-    for (int mu = 0; mu < 4; ++mu) eloc -= s[4 + mu] * e[mu];
+    for (int mu = 0; mu < 4; ++mu) eloc -= s[4 + mu] * (double)e[mu];
     if (!(eloc > 0.0) || !is_finite(eloc)) return false;
     const double mu = 2.0 * rng.drand() - 1.0;
     const double dphi = kTwoPi * rng.drand();
@@ -792,7 +825,8 @@ GRT_HD bool maybe_scatter(double s[8], double stokes[4], const StepParams& P, do
     // p_mu = g_{mu nu} w^nu = eta_{mu nu} w^nu + twoH l_mu (l.w)
     double w[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) w[q] = e[q] + dx * e[4 + q] + dy * e[8 + q] + dz * e[12 + q];
+    for (int q = 0; q < 4; ++q)
+        w[q] = (double)e[q] + dx * (double)e[4 + q] + dy * (double)e[8 + q] + dz * (double)e[12 + q];
     const double lw = w[0] + g.l1 * w[1] + g.l2 * w[2] + g.l3 * w[3];
     s[4] = eloc * (-w[0] + g.twoH * lw);
     s[5] = eloc * (w[1] + g.twoH * g.l1 * lw);
@@ -836,22 +870,50 @@ struct Photon {
     uint8_t phase, term;
 };
 
+// Restores g^{mu nu} p_mu p_nu = 0 (mixed-precision form, after a scattering
+// event) by scaling the spatial momentum (p_t, the conserved energy, is kept): one
+// Newton step on f(c) = -p_t^2 + c^2 |p|^2 - 2H (-p_t + c l.p)^2 from c = 1.
+GRT_HD void null_reproject(double s[8], const StepParams& P)
+{
+    const double r = ks_radius(s[1], s[2], s[3], P.a);
+    double l[4];
+    ks_l(s[1], s[2], s[3], r, P.a, l);
+    const double H = ks_H(r, s[3], P.a, P.m);
+    const double p0 = s[4];
+    const double pp = s[5] * s[5] + s[6] * s[6] + s[7] * s[7];
+    const double lq = l[1] * s[5] + l[2] * s[6] + l[3] * s[7];
+    const double u = lq - p0;
+    const double f = (pp - p0 * p0) - 2.0 * H * u * u;
+    const double df = 2.0 * pp - 4.0 * H * u * lq;
+    if (!(df != 0.0) || !is_finite(f) || !is_finite(df)) return;
+    const double c = 1.0 - f / df;
+    s[5] *= c;
+    s[6] *= c;
+    s[7] *= c;
+}
+
 // Everything after the RK step in Geodesic_cartesian_kerr_schild::operator()
 // (integrate_cart_ks.hpp:308-404): fluid sampling, the image-mode emission
 // event, radiative transfer, scattering and the camera test.  `phase0` is the
 // phase the call started with; `taken` the affine step just integrated.
 // On arrival at the camera (phase becomes kArrived) the screen coordinates
 // are returned in (au, av).  Returns whether the fluid was sampled.
-template <int kSep>
+//
+// kMixed (the mixed-precision form) builds the fluid tetrad and evaluates
+// the attenuation factor exp(-dtau) in FP32; neither feeds back into the
+// geodesic or its step-size sequence.  A photon scattered with the FP32
+// tetrad is put back on the null cone in FP64 (null_reproject).
+template <int kSep, int kMixed = 0>
 GRT_HD bool post_step(Photon& ph, uint8_t phase0, double taken, const GridView& G,
                       const StepParams& P, PhotonRng& rng, double& au, double& av)
 {
+    typedef typename std::conditional<kMixed != 0, float, double>::type T;
     const bool image = (P.mode == 0);
     const bool needs = P.scatter || P.absorb || P.emit || (image && phase0 == kBackward);
     FluidCellCGS fl;
-    double e[16];
+    T e[16];
     KSMetric g;
-    const bool sampled = needs && sample_fluid<kSep>(ph.s, G, P, fl, e, g);
+    const bool sampled = needs && sample_fluid<kSep, T>(ph.s, G, P, fl, e, g);
 
     if (image && phase0 == kBackward && sampled) {
         double nu = 0.0, energy = 0.0;
@@ -863,7 +925,7 @@ GRT_HD bool post_step(Photon& ph, uint8_t phase0, double taken, const GridView& 
             ph.freq = nu;
             double eem = 0.0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) eem -= ph.s[4 + q] * e[q];
+            for (int q = 0; q < 4; ++q) eem -= ph.s[4 + q] * (double)e[q];
             ph.eem = eem;
             ph.stokes[0] = energy;
             ph.stokes[1] = ph.stokes[2] = ph.stokes[3] = 0.0;
@@ -877,12 +939,15 @@ GRT_HD bool post_step(Photon& ph, uint8_t phase0, double taken, const GridView& 
     } else if (sampled && phase0 != kBackward) {
         // apply_radiative_transfer_step, radiative_transfer.hpp:81-96
         if (P.absorb) {
-            const double att = exp(-ComputeOpticalDepthIncrement(fl, ph.freq, taken, P.kabs));
+            const double dtau = ComputeOpticalDepthIncrement(fl, ph.freq, taken, P.kabs);
+            const double att = kMixed ? (double)expf((float)-dtau) : exp(-dtau);
 #pragma unroll
             for (int q = 0; q < 4; ++q) ph.stokes[q] *= att;
         }
         if (P.emit) ph.stokes[0] += ComputeEmissivityIncrement(fl, ph.freq, taken, P.kemit);
-        if (P.scatter) maybe_scatter(ph.s, ph.stokes, P, fabs(ph.dl), fl, e, g, rng);
+        if (P.scatter && maybe_scatter<T>(ph.s, ph.stokes, P, fabs(ph.dl), fl, e, g, rng) &&
+            kMixed)
+            null_reproject(ph.s, P);
     }
 
     if (image && phase0 == kForward) {
@@ -1011,7 +1076,7 @@ GRT_HD void lane_reset(Lane& L)
 // is thereby split at its retry loop, so lanes whose step is rejected retry
 // while their neighbours go on.  Returns true when the functor call
 // completed in this iteration; (au, av) are set when the photon arrived.
-template <int kIntegrator, int kSep>
+template <int kIntegrator, int kSep, class Rhs = RhsExact, int kMixed = 0>
 GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, PhotonRng& rng,
                          LaneCounters& n, double& au, double& av)
 {
@@ -1022,18 +1087,18 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
         if (kIntegrator != 1) {
             // RK4, integrate_cart_ks.hpp:269-272
             const double taken = ph.dl;
-            rk4_step(ph.s, taken, P.a, P.m);
+            rk4_step<Rhs>(ph.s, taken, P);
             ++n.accepted;
-            n.samples += post_step<kSep>(ph, L.phase0, taken, G, P, rng, au, av) ? 1u : 0u;
+            n.samples += post_step<kSep, kMixed>(ph, L.phase0, taken, G, P, rng, au, av) ? 1u : 0u;
             return true;
         }
-        ks_rhs(ph.s, P.a, P.m, L.k1);
+        Rhs::eval(ph.s, P, L.k1);
         L.attempt = 0;
         L.in_call = true;
     }
     // One attempt of the retry loop, integrate_cart_ks.hpp:273-299.
     const double taken = ph.dl;
-    const bool accepted = rk45_attempt(ph.s, &ph.dl, L.k1, P);
+    const bool accepted = rk45_attempt<Rhs, kMixed>(ph.s, &ph.dl, L.k1, P);
     ++L.attempt;
     if (accepted) {
         ++n.accepted;
@@ -1042,7 +1107,7 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
         // register allocator need not keep it alive across the post-step.
 #pragma unroll
         for (int q = 0; q < 8; ++q) L.k1[q] = 0.0;
-        n.samples += post_step<kSep>(ph, L.phase0, taken, G, P, rng, au, av) ? 1u : 0u;
+        n.samples += post_step<kSep, kMixed>(ph, L.phase0, taken, G, P, rng, au, av) ? 1u : 0u;
         return true;
     }
     ++n.rejected;

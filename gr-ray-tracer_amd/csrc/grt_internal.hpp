@@ -59,11 +59,15 @@ struct TraceLaunch {
     int step0;                // global step index of the first call of this launch
     int use_lds;              // stage the separable search tables in LDS
     int lds_doubles;          // size of that staging area
+    int mixed;                // GRT_PRECISION_MIXED: run the mixed FP32/FP64 kernel
 };
 
 // Kernel launchers (grt_kernels.hip).  They only enqueue work on `stream`.
 hipError_t launch_trace(const TraceLaunch& a, int blocks, int block, size_t lds_bytes,
                         hipStream_t stream);
+// grt_mixed.hip
+hipError_t launch_trace_mixed(const TraceLaunch& a, int blocks, int block, size_t lds_bytes,
+                              hipStream_t stream);
 hipError_t launch_build_live(const uint8_t* term, uint64_t n, uint32_t* list, uint32_t* count,
                              hipStream_t stream);
 hipError_t launch_image_products(const PhotonPtrs& ph, uint64_t n, int64_t nx, int64_t ny,

@@ -1,0 +1,61 @@
+// grt_mixed.hip -- the mixed-precision form of the tracing kernel.
+//
+// The geodesic stays FP64: the photon state, every right-hand-side
+// evaluation, the Runge-Kutta stage sums and the RK45 error norm.  The RK45
+// step-size proposal is driven by |x5 - x4|, a difference of two nearly
+// equal stage sums, so any FP32 rounding inside the stages (6e-8 relative)
+// would move the accept/reject decisions and step sizes by ~1e-3 relative at
+// rtol = 1e-7.  The image-mode emission point and the camera arrival point
+// are RK step endpoints, so such a step sequence changes most pixels.
+// FP64 is therefore kept for the geodesic, with relaxed rounding: this file
+// is compiled with FMA contraction, reciprocal-based division
+// (-freciprocal-math, and the RHS's quotients as products with four
+// reciprocals) and the hardware's approximate sqrt / rsq / rcp sequences
+// (-fapprox-func).  These change the result only at the ulp level, which
+// moves step sizes by ~1e-8 relative.
+//
+// FP32 is used where nothing feeds back into the step sequence: the fluid
+// tetrad (Gram-Schmidt under the Kerr-Schild metric), the attenuation factor
+// exp(-dtau) of the radiative-transfer update, and the controller's
+// err^(-1/4).  A photon scattered with the FP32 tetrad is put back on the
+// null cone in FP64.
+//
+// On this chip FP32 arithmetic issues at the FP64 rate unless it is packed
+// two per lane, which this build does not use; FP32 pays only where it
+// replaces FP64 division, square roots and transcendentals.
+#include <hip/hip_runtime.h>
+
+#include "grt_internal.hpp"
+#include "grt_trace.hpp"
+
+namespace grt {
+
+namespace {
+
+template <int kIntegrator, int kSep>
+__global__ __launch_bounds__(256, GRT_TRACE_WAVES_PER_EU) void trace_mixed(TraceLaunch a)
+{
+    trace_body<kIntegrator, kSep, RhsExact, 1>(a);
+}
+
+}  // namespace
+
+hipError_t launch_trace_mixed(const TraceLaunch& a, int blocks, int block, size_t lds_bytes,
+                              hipStream_t stream)
+{
+    const bool sep = a.grid.separable || !a.grid.available;
+    if (a.integrator == GRT_RK4) {
+        if (sep)
+            hipLaunchKernelGGL((trace_mixed<GRT_RK4, 1>), dim3(blocks), dim3(block), lds_bytes, stream, a);
+        else
+            hipLaunchKernelGGL((trace_mixed<GRT_RK4, 0>), dim3(blocks), dim3(block), lds_bytes, stream, a);
+    } else {
+        if (sep)
+            hipLaunchKernelGGL((trace_mixed<GRT_RK45, 1>), dim3(blocks), dim3(block), lds_bytes, stream, a);
+        else
+            hipLaunchKernelGGL((trace_mixed<GRT_RK45, 0>), dim3(blocks), dim3(block), lds_bytes, stream, a);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace grt

@@ -148,8 +148,15 @@ def default_params(**overrides):
     return p
 
 
-def empty_photons(n):
-    return {name: np.zeros(n, dtype=dt) for name, dt in PHOTON_FIELDS}
+def empty_photons(n, device=None):
+    """Zeroed photon arrays: numpy on the host, or torch tensors on `device`."""
+    if device is None:
+        return {name: np.zeros(n, dtype=dt) for name, dt in PHOTON_FIELDS}
+    import torch
+    tdt = {np.dtype(np.float64): torch.float64, np.dtype(np.uint64): torch.int64,
+           np.dtype(np.uint8): torch.uint8}
+    return {name: torch.zeros(n, dtype=tdt[np.dtype(dt)], device=device)
+            for name, dt in PHOTON_FIELDS}
 
 
 def _ptr(a):

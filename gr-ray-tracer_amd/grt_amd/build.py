@@ -17,8 +17,14 @@ CSRC = os.path.join(ROOT, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 LIB_PATH = os.path.join(ROOT, "libgrt_amd.so")
 
-SOURCES = ["grt_kernels.hip", "grt_disk.hip", "grt_api.cpp"]
-HEADERS = ["grt_device.hpp", "grt_internal.hpp"]
+SOURCES = ["grt_kernels.hip", "grt_mixed.hip", "grt_disk.hip", "grt_api.cpp"]
+HEADERS = ["grt_device.hpp", "grt_internal.hpp", "grt_trace.hpp"]
+# The mixed-precision kernel does not reproduce the reference's rounding
+# operation for operation: it may contract a * b + c into one FMA, divide by
+# multiplying with a reciprocal and use the hardware's approximate
+# sqrt / exp / reciprocal instructions.
+RELAXED_FLAGS = {"grt_mixed.hip": ["-freciprocal-math", "-fapprox-func",
+                                   "-DGRT_RHS_RECIPROCALS=1"]}
 
 
 def hipcc():
@@ -28,10 +34,10 @@ def hipcc():
     raise RuntimeError("hipcc not found: the HIP toolchain is required to build libgrt_amd.so")
 
 
-def compile_flags(resource_usage=False):
+def compile_flags(resource_usage=False, contract="off"):
     # -ffp-contract=off: the FP64 path rounds every +, -, * exactly as the
     # reference's expressions do (no fused multiply-add), see grt_device.hpp.
-    flags = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    flags = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=" + contract,
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function",
              "-I" + INCLUDE, "-I" + CSRC]
@@ -40,15 +46,21 @@ def compile_flags(resource_usage=False):
     return flags
 
 
-def build(verbose=False, resource_usage=False):
-    """Compile every source into libgrt_amd.so (always recompiles)."""
+def build(verbose=False, resource_usage=False, extra_flags=None, out=None):
+    """Compile every source into libgrt_amd.so (always recompiles).
+    extra_flags: {source: [flags]} appended for that source; out: another
+    output path for the library."""
+    extra_flags = extra_flags or {}
+    lib_path = out or LIB_PATH
     cc = hipcc()
     objs = []
-    build_dir = os.path.join(ROOT, "_build")
+    build_dir = os.path.join(ROOT, "_build") if out is None else out + ".objs"
     os.makedirs(build_dir, exist_ok=True)
     for src in SOURCES:
         obj = os.path.join(build_dir, src + ".o")
-        cmd = [cc] + compile_flags(resource_usage and src.endswith(".hip")) + \
+        contract = "fast" if src in RELAXED_FLAGS else "off"
+        cmd = [cc] + compile_flags(resource_usage and src.endswith(".hip"), contract) + \
+            RELAXED_FLAGS.get(src, []) + extra_flags.get(src, []) + \
             ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
@@ -59,14 +71,14 @@ def build(verbose=False, resource_usage=False):
         if verbose or resource_usage:
             sys.stderr.write(r.stderr)
         objs.append(obj)
-    tmp = LIB_PATH + ".tmp"
+    tmp = lib_path + ".tmp"
     cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc link of libgrt_amd.so failed")
-    os.replace(tmp, LIB_PATH)
-    return LIB_PATH
+    os.replace(tmp, lib_path)
+    return lib_path
 
 
 if __name__ == "__main__":
