@@ -675,7 +675,7 @@ GRT_HD bool fluid_tetrad(const KSMetric& g, const double u_in[4], T e[16])
 // kSep selects the separable fast search at compile time.
 template <int kSep, class T>
 GRT_HD bool sample_fluid(const double s[8], const GridView& G, const StepParams& P,
-                         FluidCellCGS& fl, T e[16], KSMetric& g)
+                         FluidCellCGS& fl, T e[16], KSMetric& g, uint32_t& cell)
 {
     if (!G.available) return false;
     const double radius = ks_radius(s[1], s[2], s[3], P.a);
@@ -694,7 +694,8 @@ GRT_HD bool sample_fluid(const double s[8], const GridView& G, const StepParams&
         found = locate_general(G, radius, polar, az, i, j, k);
     }
     if (!found) return false;
-    const double* c = G.cells + (((size_t)i * G.nth + j) * G.nph + k) * kCellDoubles;
+    cell = (uint32_t)(((size_t)i * G.nth + j) * G.nph + k);
+    const double* c = G.cells + (size_t)cell * kCellDoubles;
     fl.position_cm[0] = s[1] * P.length_scale;
     fl.position_cm[1] = s[2] * P.length_scale;
     fl.position_cm[2] = s[3] * P.length_scale;
@@ -863,6 +864,15 @@ GRT_HD double observer_energy(const StepParams& P, const double s[8])
 // Per-photon state and the post-step part of the functor
 // ---------------------------------------------------------------------------
 
+// Observer of the per-call events the differentiable form needs: every
+// radiative-transfer update (cell index, fluid, frequency, affine step and
+// Stokes I before the update) and every event that sets Stokes I regardless
+// of its previous value (emission event, zeroing).  NoHook compiles away.
+struct NoHook {
+    GRT_HD void rt(const FluidCellCGS&, uint32_t, double, double, double) {}
+    GRT_HD void reset() {}
+};
+
 struct Photon {
     double s[8];
     double stokes[4];
@@ -903,9 +913,9 @@ GRT_HD void null_reproject(double s[8], const StepParams& P)
 // the attenuation factor exp(-dtau) in FP32; neither feeds back into the
 // geodesic or its step-size sequence.  A photon scattered with the FP32
 // tetrad is put back on the null cone in FP64 (null_reproject).
-template <int kSep, int kMixed = 0>
+template <int kSep, int kMixed = 0, class Hook = NoHook>
 GRT_HD bool post_step(Photon& ph, uint8_t phase0, double taken, const GridView& G,
-                      const StepParams& P, PhotonRng& rng, double& au, double& av)
+                      const StepParams& P, PhotonRng& rng, double& au, double& av, Hook& hook)
 {
     typedef typename std::conditional<kMixed != 0, float, double>::type T;
     const bool image = (P.mode == 0);
@@ -913,7 +923,8 @@ GRT_HD bool post_step(Photon& ph, uint8_t phase0, double taken, const GridView& 
     FluidCellCGS fl;
     T e[16];
     KSMetric g;
-    const bool sampled = needs && sample_fluid<kSep, T>(ph.s, G, P, fl, e, g);
+    uint32_t cell = 0;
+    const bool sampled = needs && sample_fluid<kSep, T>(ph.s, G, P, fl, e, g, cell);
 
     if (image && phase0 == kBackward && sampled) {
         double nu = 0.0, energy = 0.0;
@@ -936,7 +947,9 @@ GRT_HD bool post_step(Photon& ph, uint8_t phase0, double taken, const GridView& 
             ph.term = 1;
             ph.phase = kRejected;
         }
+        hook.reset();
     } else if (sampled && phase0 != kBackward) {
+        if (P.absorb || P.emit) hook.rt(fl, cell, ph.freq, taken, ph.stokes[0]);
         // apply_radiative_transfer_step, radiative_transfer.hpp:81-96
         if (P.absorb) {
             const double dtau = ComputeOpticalDepthIncrement(fl, ph.freq, taken, P.kabs);
@@ -955,14 +968,17 @@ GRT_HD bool post_step(Photon& ph, uint8_t phase0, double taken, const GridView& 
         if (crossing > 0) {
             const double eobs = observer_energy(P, ph.s);
             const double eem = ph.eem;
-            if (eobs > 0.0 && eem > 0.0 && is_finite(eobs) && is_finite(eem))
+            if (eobs > 0.0 && eem > 0.0 && is_finite(eobs) && is_finite(eem)) {
                 ph.freq *= eobs / eem;
-            else
+            } else {
                 ph.stokes[0] = ph.stokes[1] = ph.stokes[2] = ph.stokes[3] = 0.0;
+                hook.reset();
+            }
             ph.phase = kArrived;
             ph.term = 1;
         } else if (crossing < 0) {
             ph.stokes[0] = ph.stokes[1] = ph.stokes[2] = ph.stokes[3] = 0.0;
+            hook.reset();
             ph.phase = kRejected;
             ph.term = 1;
         }
@@ -972,11 +988,13 @@ GRT_HD bool post_step(Photon& ph, uint8_t phase0, double taken, const GridView& 
 
 // The radius tests that open every functor call (integrate_cart_ks.hpp:248-263).
 // Returns true when the photon terminated.
-GRT_HD bool radius_exit(Photon& ph, const StepParams& P)
+template <class Hook>
+GRT_HD bool radius_exit(Photon& ph, const StepParams& P, Hook& hook)
 {
     const double dist = ks_radius(ph.s[1], ph.s[2], ph.s[3], P.a);
     if (dist <= P.r_min) {
         ph.stokes[0] = ph.stokes[1] = ph.stokes[2] = ph.stokes[3] = 0.0;
+        hook.reset();
         ph.term = 1;
         ph.phase = kRejected;
         return true;
@@ -1076,20 +1094,20 @@ GRT_HD void lane_reset(Lane& L)
 // is thereby split at its retry loop, so lanes whose step is rejected retry
 // while their neighbours go on.  Returns true when the functor call
 // completed in this iteration; (au, av) are set when the photon arrived.
-template <int kIntegrator, int kSep, class Rhs = RhsExact, int kMixed = 0>
+template <int kIntegrator, int kSep, class Rhs = RhsExact, int kMixed = 0, class Hook = NoHook>
 GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, PhotonRng& rng,
-                         LaneCounters& n, double& au, double& av)
+                         LaneCounters& n, double& au, double& av, Hook& hook)
 {
     Photon& ph = L.ph;
     if (!L.in_call) {
         L.phase0 = ph.phase;
-        if (radius_exit(ph, P)) return true;
+        if (radius_exit(ph, P, hook)) return true;
         if (kIntegrator != 1) {
             // RK4, integrate_cart_ks.hpp:269-272
             const double taken = ph.dl;
             rk4_step<Rhs>(ph.s, taken, P);
             ++n.accepted;
-            n.samples += post_step<kSep, kMixed>(ph, L.phase0, taken, G, P, rng, au, av) ? 1u : 0u;
+            n.samples += post_step<kSep, kMixed>(ph, L.phase0, taken, G, P, rng, au, av, hook) ? 1u : 0u;
             return true;
         }
         Rhs::eval(ph.s, P, L.k1);
@@ -1107,7 +1125,7 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
         // register allocator need not keep it alive across the post-step.
 #pragma unroll
         for (int q = 0; q < 8; ++q) L.k1[q] = 0.0;
-        n.samples += post_step<kSep, kMixed>(ph, L.phase0, taken, G, P, rng, au, av) ? 1u : 0u;
+        n.samples += post_step<kSep, kMixed>(ph, L.phase0, taken, G, P, rng, au, av, hook) ? 1u : 0u;
         return true;
     }
     ++n.rejected;
@@ -1118,6 +1136,14 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
         return true;
     }
     return false;
+}
+
+template <int kIntegrator, int kSep, class Rhs = RhsExact, int kMixed = 0>
+GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, PhotonRng& rng,
+                         LaneCounters& n, double& au, double& av)
+{
+    NoHook h;
+    return lane_iterate<kIntegrator, kSep, Rhs, kMixed, NoHook>(L, G, P, rng, n, au, av, h);
 }
 
 }  // namespace grt

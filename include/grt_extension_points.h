@@ -12,6 +12,13 @@
  * The bodies below are the reference defaults.  Kokkos:: math becomes the
  * plain HIP device math functions (fmax, fabs, exp, ...).
  *
+ * The differentiable form (grt_trace_taped / grt_adjoint) also needs the
+ * partial derivatives of the two radiative-transfer increments; they come
+ * from the two ...Gradient companions below, which have no counterpart in
+ * the reference.  Whoever edits an increment edits its companion too; the
+ * test suite checks each companion against central differences of its
+ * increment.
+ *
  * `real` is double in the FP64 path.  The RNG argument of
  * GeneratePhotonsCustom offers `double drand()` returning a uniform [0, 1)
  * draw from the per-photon counter-based stream.
@@ -56,6 +63,41 @@ __host__ __device__ inline double ComputeEmissivityIncrement(const FluidCellCGS&
     (void)frequency_hz;
     return fmax(0.0, emission_coefficient * fluid.density_g_cm3 * fluid.temperature_k *
                          fabs(path_length_step));
+}
+
+/* dtau and its partial derivatives with respect to fluid.density_g_cm3,
+ * fluid.temperature_k and absorption_coefficient (zero where the default
+ * body's fmax(0, .) clamps). */
+__host__ __device__ inline double ComputeOpticalDepthIncrementGradient(
+    const FluidCellCGS& fluid, double frequency_hz, double path_length_step,
+    double absorption_coefficient, double* d_density, double* d_temperature,
+    double* d_coefficient)
+{
+    const double dtau = ComputeOpticalDepthIncrement(fluid, frequency_hz, path_length_step,
+                                                     absorption_coefficient);
+    const double len = fabs(path_length_step);
+    const bool live = dtau > 0.0;
+    *d_density = live ? absorption_coefficient * len : 0.0;
+    *d_temperature = 0.0;
+    *d_coefficient = live ? fluid.density_g_cm3 * len : 0.0;
+    return dtau;
+}
+
+/* dI and its partial derivatives with respect to fluid.density_g_cm3,
+ * fluid.temperature_k and emission_coefficient. */
+__host__ __device__ inline double ComputeEmissivityIncrementGradient(
+    const FluidCellCGS& fluid, double frequency_hz, double path_length_step,
+    double emission_coefficient, double* d_density, double* d_temperature,
+    double* d_coefficient)
+{
+    const double dI = ComputeEmissivityIncrement(fluid, frequency_hz, path_length_step,
+                                                 emission_coefficient);
+    const double len = fabs(path_length_step);
+    const bool live = dI > 0.0;
+    *d_density = live ? emission_coefficient * fluid.temperature_k * len : 0.0;
+    *d_temperature = live ? emission_coefficient * fluid.density_g_cm3 * len : 0.0;
+    *d_coefficient = live ? fluid.density_g_cm3 * fluid.temperature_k * len : 0.0;
+    return dI;
 }
 
 /* Superphotons per cell from the density in code units.  The reference's

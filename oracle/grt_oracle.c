@@ -24,6 +24,9 @@
  *   camera / image products ...... flat-space (M = 0) straight rays hit the
  *                                  analytic screen point; test_output_stride.cpp
  *   Philox4x32-10 ................ Random123 known-answer vectors
+ *   reference adjoint ............ central finite differences of orc_trace
+ *                                  (tests/test_oracle_adjoint.py); the increments'
+ *                                  derivatives against central differences
  *   RNG-driven branches vs the reference's XorShift64 pool: parity unpinned
  *   (statistical only).
  */
@@ -582,7 +585,8 @@ static void fluid_direction_to_momentum(const double X[4], const double e[16], d
 }
 
 /* scattering.hpp:185-251 */
-int orc_sample_fluid(const double s[8], const orc_grid *g, orc_fluid *f, double e[16])
+static int sample_fluid_cell(const double s[8], const orc_grid *g, orc_fluid *f, double e[16],
+                             size_t *cell_out)
 {
     if (!g->available) return 0;
     const double radius = orc_compute_r(s[1], s[2], s[3], g->spin);
@@ -595,6 +599,7 @@ int orc_sample_fluid(const double s[8], const orc_grid *g, orc_fluid *f, double 
     if (!orc_locate_cell(g, radius, polar, az, ijk)) return 0;
     const int i = ijk[0], j = ijk[1], k = ijk[2];
     const size_t cell = ((size_t)i * g->nth + j) * g->nph + k;
+    *cell_out = cell;
     f->position_cm[0] = s[1] * g->length_scale;
     f->position_cm[1] = s[2] * g->length_scale;
     f->position_cm[2] = s[3] * g->length_scale;
@@ -622,6 +627,12 @@ int orc_sample_fluid(const double s[8], const orc_grid *g, orc_fluid *f, double 
     return 1;
 }
 
+int orc_sample_fluid(const double s[8], const orc_grid *g, orc_fluid *f, double e[16])
+{
+    size_t cell;
+    return sample_fluid_cell(s, g, f, e, &cell);
+}
+
 /* ------------------------------------------------------------------------- */
 /* Radiative transfer: radiative_transfer.hpp:54-96                           */
 /* ------------------------------------------------------------------------- */
@@ -638,6 +649,30 @@ double orc_emissivity_increment(const orc_fluid *f, double nu, double dl, double
 {
     (void)nu;
     return fmax(0.0, coeff * f->density_g_cm3 * f->temperature_k * fabs(dl));
+}
+
+/* Partial derivatives of the two default increments with respect to the
+ * density, the temperature and the coefficient (zero where fmax clamps). */
+double orc_optical_depth_increment_grad(const orc_fluid *f, double nu, double dl, double coeff,
+                                        double d[3])
+{
+    const double v = orc_optical_depth_increment(f, nu, dl, coeff);
+    const int live = v > 0.0;
+    d[0] = live ? coeff * fabs(dl) : 0.0;
+    d[1] = 0.0;
+    d[2] = live ? f->density_g_cm3 * fabs(dl) : 0.0;
+    return v;
+}
+
+double orc_emissivity_increment_grad(const orc_fluid *f, double nu, double dl, double coeff,
+                                     double d[3])
+{
+    const double v = orc_emissivity_increment(f, nu, dl, coeff);
+    const int live = v > 0.0;
+    d[0] = live ? coeff * f->temperature_k * fabs(dl) : 0.0;
+    d[1] = live ? coeff * f->density_g_cm3 * fabs(dl) : 0.0;
+    d[2] = live ? f->density_g_cm3 * f->temperature_k * fabs(dl) : 0.0;
+    return v;
 }
 
 /* radiative_transfer.hpp:81-96 */
@@ -809,8 +844,38 @@ double orc_camera_observer_energy(const orc_params *p, const double s[8])
 /* The per-photon functor: integrate_cart_ks.hpp:231-419                      */
 /* ------------------------------------------------------------------------- */
 
+/* Per-photon record of the events the adjoint needs (reference adjoint). */
+typedef struct {
+    int kind;              /* 0: radiative-transfer update, 1: I set regardless of its past */
+    size_t cell;
+    double I_before, taken, nu;
+    orc_fluid fluid;
+} tape_entry;
+
+typedef struct {
+    tape_entry *e;
+    int64_t n, cap;
+} photon_tape;
+
+static void tape_push(photon_tape *t, int kind, size_t cell, double I_before, double taken,
+                      double nu, const orc_fluid *f)
+{
+    if (!t) return;
+    if (t->n == t->cap) {
+        t->cap = t->cap ? 2 * t->cap : 16;
+        t->e = (tape_entry *)realloc(t->e, (size_t)t->cap * sizeof(tape_entry));
+    }
+    tape_entry *e = &t->e[t->n++];
+    e->kind = kind;
+    e->cell = cell;
+    e->I_before = I_before;
+    e->taken = taken;
+    e->nu = nu;
+    if (f) e->fluid = *f; else memset(&e->fluid, 0, sizeof e->fluid);
+}
+
 static void step_photon(const orc_params *p, const orc_grid *g, orc_photons *ph, int64_t idx,
-                        uint32_t *rng_draw, orc_stats *st)
+                        uint32_t *rng_draw, orc_stats *st, photon_tape *tape)
 {
     if (ph->terminate[idx]) return;
     st->functor_calls++;
@@ -820,6 +885,7 @@ static void step_photon(const orc_params *p, const orc_grid *g, orc_photons *ph,
     const double dist = orc_compute_r(s[1], s[2], s[3], p->a_bh);
     if (dist <= p->r_term_min) {
         ph->I[idx] = ph->Q[idx] = ph->U[idx] = ph->V[idx] = 0.0;
+        tape_push(tape, 1, 0, 0.0, 0.0, 0.0, NULL);
         ph->terminate[idx] = 1;
         ph->phase[idx] = PH_REJECTED;
         return;
@@ -865,9 +931,10 @@ static void step_photon(const orc_params *p, const orc_grid *g, orc_photons *ph,
     const int needs = p->scattering_enabled || p->absorption_enabled || p->emission_enabled ||
                       (image && phase == PH_BACKWARD);
     int sampled = 0;
+    size_t cell = 0;
     if (needs) {
         st->fluid_samples++;
-        sampled = orc_sample_fluid(s, g, &fluid, e);
+        sampled = sample_fluid_cell(s, g, &fluid, e, &cell);
     }
     orc_rng rng = {ph->id[idx], rng_draw ? rng_draw[idx] : 0u, p->rng_stream << 1, p->seed};
 
@@ -891,7 +958,10 @@ static void step_photon(const orc_params *p, const orc_grid *g, orc_photons *ph,
             ph->terminate[idx] = 1;
             ph->phase[idx] = PH_REJECTED;
         }
+        tape_push(tape, 1, 0, 0.0, 0.0, 0.0, NULL);
     } else if (sampled && phase != PH_BACKWARD) {
+        if (p->absorption_enabled || p->emission_enabled)
+            tape_push(tape, 0, cell, stokes[0], taken, ph->frequency[idx], &fluid);
         if (p->absorption_enabled || p->emission_enabled)
             orc_apply_radiative_transfer(stokes, &fluid, ph->frequency[idx], taken,
                                          p->absorption_enabled, p->emission_enabled,
@@ -908,16 +978,19 @@ static void step_photon(const orc_params *p, const orc_grid *g, orc_photons *ph,
         if (crossing > 0) {
             const double eobs = orc_camera_observer_energy(p, s);
             const double eem = ph->emission_frame_energy[idx];
-            if (eobs > 0.0 && eem > 0.0 && isfinite(eobs) && isfinite(eem))
+            if (eobs > 0.0 && eem > 0.0 && isfinite(eobs) && isfinite(eem)) {
                 ph->frequency[idx] *= eobs / eem;
-            else
+            } else {
                 stokes[0] = stokes[1] = stokes[2] = stokes[3] = 0.0;
+                tape_push(tape, 1, 0, 0.0, 0.0, 0.0, NULL);
+            }
             ph->theta_disp[idx] = au;
             ph->phi_disp[idx] = av;
             ph->phase[idx] = PH_ARRIVED;
             ph->terminate[idx] = 1;
         } else if (crossing < 0) {
             stokes[0] = stokes[1] = stokes[2] = stokes[3] = 0.0;
+            tape_push(tape, 1, 0, 0.0, 0.0, 0.0, NULL);
             ph->phase[idx] = PH_REJECTED;
             ph->terminate[idx] = 1;
         }
@@ -930,7 +1003,7 @@ static void step_photon(const orc_params *p, const orc_grid *g, orc_photons *ph,
 void orc_step_photon(const orc_params *p, const orc_grid *g, orc_photons *ph, int64_t idx,
                      uint32_t *rng_draw, orc_stats *st)
 {
-    step_photon(p, g, ph, idx, rng_draw, st);
+    step_photon(p, g, ph, idx, rng_draw, st, NULL);
 }
 
 /* Host loop, integrate_cart_ks.hpp:423-535 (single rank, no output files):
@@ -938,9 +1011,9 @@ void orc_step_photon(const orc_params *p, const orc_grid *g, orc_photons *ph, in
  * fraction is recounted only on output steps (step % interval == 0 or the last
  * step) and the loop stops once it reaches termination_percent.  Global steps
  * on which no photon is live are skipped without changing any state. */
-void orc_trace(const orc_params *p_in, const orc_grid *g, orc_photons *ph, int64_t n,
-               int32_t max_steps, double termination_percent, int64_t output_interval,
-               int32_t nthreads, orc_stats *st)
+static void trace_impl(const orc_params *p_in, const orc_grid *g, orc_photons *ph, int64_t n,
+                       int32_t max_steps, double termination_percent, int64_t output_interval,
+                       int32_t nthreads, orc_stats *st, photon_tape *tapes)
 {
     orc_params p = *p_in;
     const int avail = g && g->available && g->nr > 0 && g->nth > 0 && g->nph > 0;
@@ -969,7 +1042,7 @@ void orc_trace(const orc_params *p_in, const orc_grid *g, orc_photons *ph, int64
             for (int64_t i = 0; i < n; ++i) {
                 orc_stats ls;
                 memset(&ls, 0, sizeof ls);
-                step_photon(&p, &gl, ph, i, draws, &ls);
+                step_photon(&p, &gl, ph, i, draws, &ls, tapes ? &tapes[i] : NULL);
                 calls += ls.functor_calls; acc += ls.accepted_steps;
                 rej += ls.rejected_attempts; fail += ls.failed_steps; samp += ls.fluid_samples;
                 nlive += ph->terminate[i] ? 0 : 1;
@@ -985,6 +1058,62 @@ void orc_trace(const orc_params *p_in, const orc_grid *g, orc_photons *ph, int64
     }
     st->terminated = (uint64_t)(n - live);
     free(draws);
+}
+
+void orc_trace(const orc_params *p_in, const orc_grid *g, orc_photons *ph, int64_t n,
+               int32_t max_steps, double termination_percent, int64_t output_interval,
+               int32_t nthreads, orc_stats *st)
+{
+    trace_impl(p_in, g, ph, n, max_steps, termination_percent, output_interval, nthreads, st,
+               NULL);
+}
+
+/* Reference gradient of L(I_final) for the differentiable form: traces the
+ * photons like orc_trace while recording every radiative-transfer update and
+ * every event that sets Stokes I regardless of its past, then walks each
+ * photon's record backwards through I <- a I + b, a = exp(-dtau), b = dI:
+ *   dL/dI_before = a dL/dI_after,
+ *   dL/dtheta   += dL/dI_after (-a I_before ddtau/dtheta + d(dI)/dtheta)
+ * for theta = the sampled cell's density and temperature (chained to code
+ * units through the unit scales) and the two coefficients.  The emission
+ * energy E/int(rho norm) is piecewise constant in rho and is held fixed.
+ * grad_rho and grad_T (one per cell) and grad_coeff (absorption, emission)
+ * are overwritten.  No checkpointing: the record holds every update. */
+void orc_trace_adjoint(const orc_params *p_in, const orc_grid *g, orc_photons *ph, int64_t n,
+                       int32_t max_steps, double termination_percent, int64_t output_interval,
+                       int32_t nthreads, const double *dL_dI, double *grad_rho, double *grad_T,
+                       double grad_coeff[2], orc_stats *st)
+{
+    photon_tape *tapes = (photon_tape *)calloc((size_t)(n > 0 ? n : 1), sizeof(photon_tape));
+    trace_impl(p_in, g, ph, n, max_steps, termination_percent, output_interval, nthreads, st,
+               tapes);
+    const size_t G = (size_t)g->nr * g->nth * g->nph;
+    memset(grad_rho, 0, G * sizeof(double));
+    memset(grad_T, 0, G * sizeof(double));
+    grad_coeff[0] = grad_coeff[1] = 0.0;
+    const int absorb = p_in->absorption_enabled, emit = p_in->emission_enabled;
+    for (int64_t i = 0; i < n; ++i) {
+        double lam = dL_dI[i];
+        for (int64_t q = tapes[i].n - 1; q >= 0 && lam != 0.0; --q) {
+            const tape_entry *e = &tapes[i].e[q];
+            if (e->kind == 1) break;
+            double a = 1.0, dt[3] = {0.0, 0.0, 0.0}, di[3] = {0.0, 0.0, 0.0};
+            if (absorb)
+                a = exp(-orc_optical_depth_increment_grad(&e->fluid, e->nu, e->taken,
+                                                          p_in->absorption_coefficient, dt));
+            if (emit)
+                orc_emissivity_increment_grad(&e->fluid, e->nu, e->taken,
+                                              p_in->emission_coefficient, di);
+            const double aI = a * e->I_before;
+            grad_rho[e->cell] += lam * (di[0] - aI * dt[0]) * g->density_scale;
+            grad_T[e->cell] += lam * (di[1] - aI * dt[1]) * g->temperature_scale;
+            grad_coeff[0] += lam * (-aI * dt[2]);
+            grad_coeff[1] += lam * di[2];
+            lam *= a;
+        }
+        free(tapes[i].e);
+    }
+    free(tapes);
 }
 
 /* ------------------------------------------------------------------------- */

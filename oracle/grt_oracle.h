@@ -123,6 +123,10 @@ int orc_sample_fluid(const double state[8], const orc_grid *g, orc_fluid *fluid,
 /* ---- radiative transfer, radiative_transfer.hpp:54-96 ---- */
 double orc_optical_depth_increment(const orc_fluid *f, double nu, double dl, double coeff);
 double orc_emissivity_increment(const orc_fluid *f, double nu, double dl, double coeff);
+double orc_optical_depth_increment_grad(const orc_fluid *f, double nu, double dl, double coeff,
+                                        double d[3]);
+double orc_emissivity_increment_grad(const orc_fluid *f, double nu, double dl, double coeff,
+                                     double d[3]);
 void orc_apply_radiative_transfer(double stokes[4], const orc_fluid *f, double nu, double dl,
                                   int absorb, int emit, double kabs, double kemit);
 
@@ -148,6 +152,10 @@ void orc_step_photon(const orc_params *p, const orc_grid *g, orc_photons *ph, in
 void orc_trace(const orc_params *p, const orc_grid *g, orc_photons *ph, int64_t n,
                int32_t max_steps, double termination_percent, int64_t output_interval,
                int32_t nthreads, orc_stats *st);
+void orc_trace_adjoint(const orc_params *p, const orc_grid *g, orc_photons *ph, int64_t n,
+                       int32_t max_steps, double termination_percent, int64_t output_interval,
+                       int32_t nthreads, const double *dL_dI, double *grad_rho, double *grad_T,
+                       double grad_coeff[2], orc_stats *st);
 
 /* ---- initialisation and observation products ---- */
 void orc_init_image_photon(const orc_params *p, double screen_u, double screen_v,

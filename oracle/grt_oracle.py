@@ -139,6 +139,12 @@ def lib():
             "orc_step_photon": (None, [P, G, PH, C.c_int64, C.POINTER(C.c_uint32), S]),
             "orc_trace": (None, [P, G, PH, C.c_int64, C.c_int32, C.c_double, C.c_int64,
                                  C.c_int32, S]),
+            "orc_trace_adjoint": (None, [P, G, PH, C.c_int64, C.c_int32, C.c_double, C.c_int64,
+                                         C.c_int32, _D, _D, _D, _D, S]),
+            "orc_optical_depth_increment_grad": (C.c_double, [F, C.c_double, C.c_double,
+                                                              C.c_double, _D]),
+            "orc_emissivity_increment_grad": (C.c_double, [F, C.c_double, C.c_double,
+                                                           C.c_double, _D]),
             "orc_init_image_photon": (None, [P, C.c_double, C.c_double, C.c_double, _D, _D]),
             "orc_init_image": (None, [P, _D, _D, C.c_int64, C.c_uint64, C.c_double, PH]),
             "orc_init_disk": (None, [P, G, C.c_int64, C.c_uint64, C.c_double, PH,
@@ -242,6 +248,36 @@ def trace(params, grid_holder, ph, max_steps, termination_percent=1.0, output_in
     lib().orc_trace(C.byref(params), C.byref(g), C.byref(photon_struct(ph)), n,
                     max_steps, termination_percent, output_interval, nthreads, C.byref(st))
     return {name: getattr(st, name) for name, _ in Stats._fields_}
+
+
+def trace_adjoint(params, grid_holder, ph, max_steps, dL_dI, termination_percent=1.0,
+                  output_interval=0, nthreads=0):
+    """Trace like trace() and return (stats, dL/drho, dL/dTgas, [dL/dkappa_abs,
+    dL/dkappa_emit]) for the loss whose gradient with respect to each photon's
+    final Stokes I is dL_dI (the reference gradient, no checkpointing)."""
+    n = len(ph["x0"])
+    st = Stats()
+    g = grid_holder.grid
+    shape = (g.nr, g.nth, g.nph)
+    gr = np.zeros(shape)
+    gt = np.zeros(shape)
+    gc = np.zeros(2)
+    seeds = np.ascontiguousarray(dL_dI, dtype=np.float64)
+    assert seeds.shape == (n,)
+    lib().orc_trace_adjoint(C.byref(params), C.byref(g), C.byref(photon_struct(ph)), n,
+                            max_steps, termination_percent, output_interval, nthreads,
+                            dptr(seeds), dptr(gr), dptr(gt), dptr(gc), C.byref(st))
+    return {name: getattr(st, name) for name, _ in Stats._fields_}, gr, gt, gc
+
+
+def increment_grads(fluid, nu, dl, kabs, kemit):
+    """(dtau, [d/drho, d/dT, d/dkappa]), (dI, [...]) of the default increments."""
+    f = Fluid(**fluid) if isinstance(fluid, dict) else fluid
+    da = np.zeros(3)
+    de = np.zeros(3)
+    t = lib().orc_optical_depth_increment_grad(C.byref(f), nu, dl, kabs, dptr(da))
+    e = lib().orc_emissivity_increment_grad(C.byref(f), nu, dl, kemit, dptr(de))
+    return (t, da), (e, de)
 
 
 def init_image(params, su, sv, initial_step, id_offset=0):

@@ -1,0 +1,111 @@
+"""The oracle's reference gradient of the differentiable form.
+
+The reference adjoint (orc_trace_adjoint) is pinned by central finite
+differences of the oracle's own forward trace: a fixed set of disk-mode
+photons (emitted inside the torus, so radiative transfer acts along their
+whole path through the grid) is traced under perturbed densities,
+temperatures and coefficients, and L = sum_i w_i I_i is differenced.  The
+derivatives of the two default increments are pinned the same way.
+"""
+import numpy as np
+import pytest
+
+from conftest import c1_params
+
+H = 1e-6
+
+
+def small_torus(floor=1e-6):
+    from grt_amd.fm_torus import fishbone_moncrief
+    return fishbone_moncrief(nr=24, ntheta=12, nphi=12, r_min=1.5, r_max=60.0, mass=1.0,
+                             spin=0.9, r_in=6.0, r_pressure_max=12.0, rho_floor=floor)
+
+
+def disk_setup(oracle, fields, kabs=0.05, kemit=0.3, integrator=1):
+    kw = c1_params(mode=1, superphoton_count_normalization=6.0, absorption_coefficient=kabs,
+                   emission_coefficient=kemit, integrator=integrator, seed=7)
+    po = oracle.make_params(**kw)
+    G = oracle.GridHolder(fields, 0.9, 1.0)
+    ph0 = oracle.init_disk(po, G, 0.1)
+    return kw, po, ph0
+
+
+def loss(oracle, fields, kw, ph0, w, max_steps=4000):
+    po = oracle.make_params(**kw)
+    G = oracle.GridHolder(fields, 0.9, 1.0)
+    ph = {k: v.copy() for k, v in ph0.items()}
+    oracle.trace(po, G, ph, max_steps, 1.0, 0, 8)
+    return float(np.dot(w, ph["I"])), ph
+
+
+def test_increment_gradients_match_differences(oracle):
+    rng = np.random.default_rng(4)
+    for _ in range(20):
+        f = dict(density_g_cm3=rng.uniform(0.1, 3.0), temperature_k=rng.uniform(0.5, 4.0))
+        nu, dl = 1e12, rng.uniform(-2, 2)
+        ka, ke = rng.uniform(0.1, 2.0), rng.uniform(0.1, 2.0)
+        (t, dt), (e, de) = oracle.increment_grads(f, nu, dl, ka, ke)
+        for idx, key in ((0, "density_g_cm3"), (1, "temperature_k")):
+            fp, fm = dict(f), dict(f)
+            fp[key] *= 1 + H
+            fm[key] *= 1 - H
+            (tp, _), (ep, _) = oracle.increment_grads(fp, nu, dl, ka, ke)
+            (tm, _), (em, _) = oracle.increment_grads(fm, nu, dl, ka, ke)
+            step = 2 * H * f[key]
+            assert dt[idx] == pytest.approx((tp - tm) / step, rel=1e-7, abs=1e-12)
+            assert de[idx] == pytest.approx((ep - em) / step, rel=1e-7, abs=1e-12)
+        (tp, _), (ep, _) = oracle.increment_grads(f, nu, dl, ka * (1 + H), ke * (1 + H))
+        (tm, _), (em, _) = oracle.increment_grads(f, nu, dl, ka * (1 - H), ke * (1 - H))
+        assert dt[2] == pytest.approx((tp - tm) / (2 * H * ka), rel=1e-7)
+        assert de[2] == pytest.approx((ep - em) / (2 * H * ke), rel=1e-7)
+
+
+def test_reference_adjoint_matches_finite_differences(oracle):
+    fields = small_torus()
+    kw, po, ph0 = disk_setup(oracle, fields)
+    n = len(ph0["x0"])
+    assert 50 < n < 5000
+    rng = np.random.default_rng(9)
+    w = rng.uniform(0.5, 1.5, n)
+    G = oracle.GridHolder(fields, 0.9, 1.0)
+    ph = {k: v.copy() for k, v in ph0.items()}
+    st, gr, gt, gc = oracle.trace_adjoint(po, G, ph, 4000, w, nthreads=8)
+    L0, ph_ref = loss(oracle, fields, kw, ph0, w)
+    # the recording trace is the plain trace
+    assert np.array_equal(ph["I"], ph_ref["I"]) and np.array_equal(ph["x1"], ph_ref["x1"])
+    assert np.count_nonzero(gr) > 10 and np.count_nonzero(gt) > 10
+    # the most sensitive cells and a few random touched ones
+    touched = np.flatnonzero(gr)
+    pick = list(np.argsort(-np.abs(gr.ravel()))[:4]) + list(rng.choice(touched, 4, replace=False))
+    for c in pick:
+        for name, grad in (("rho", gr), ("Tgas", gt)):
+            fp = {k: v.copy() for k, v in fields.items()}
+            fm = {k: v.copy() for k, v in fields.items()}
+            v0 = fields[name].ravel()[c]
+            fp[name].ravel()[c] = v0 * (1 + H)
+            fm[name].ravel()[c] = v0 * (1 - H)
+            fd = (loss(oracle, fp, kw, ph0, w)[0] - loss(oracle, fm, kw, ph0, w)[0]) / (2 * H * v0)
+            g = grad.ravel()[c]
+            assert g == pytest.approx(fd, rel=1e-5, abs=1e-9 * abs(L0)), (name, c)
+    for i, key in enumerate(("absorption_coefficient", "emission_coefficient")):
+        k0 = kw[key]
+        Lp = loss(oracle, fields, dict(kw, **{key: k0 * (1 + H)}), ph0, w)[0]
+        Lm = loss(oracle, fields, dict(kw, **{key: k0 * (1 - H)}), ph0, w)[0]
+        assert gc[i] == pytest.approx((Lp - Lm) / (2 * H * k0), rel=1e-5), key
+
+
+def test_image_mode_intensity_is_independent_of_transfer(oracle):
+    """In image mode the forward pass retraces the backward path from the
+    emission cell, whose earlier step endpoints all lay outside the fluid, so
+    no radiative-transfer update happens: the reference gradient with
+    respect to rho, Tgas and both coefficients is exactly zero."""
+    from conftest import c1_fields, pixel_screen
+    fields = c1_fields()
+    kw = c1_params(theta=np.pi / 4)
+    po = oracle.make_params(**kw)
+    G = oracle.GridHolder(fields, 0.9, 1.0)
+    su, sv = pixel_screen(16)
+    ph = oracle.init_image(po, su, sv, 0.1)
+    st, gr, gt, gc = oracle.trace_adjoint(po, G, ph, 10000, np.ones(len(su)), nthreads=8)
+    assert np.all(ph["phase"] == 3)
+    assert not gr.any() and not gt.any() and not gc.any()
