@@ -147,6 +147,90 @@ def precision_leg(args, log):
     }
 
 
+def adjoint_leg(args, log):
+    """Cost of the differentiable form: forward render vs taped render +
+    adjoint, on the torus a = 0.9 (r_in = 6, r_pmax = 12) on a 128 x 64 x 64
+    grid with RK4 at a fixed affine step of 0.05 M, absorption and emission
+    on, scattering off, loss L = ||I - 0.9 I_0||_2 over the photons.
+    (1) disk-mode photons emitted in the torus (radiative transfer acts on
+    every step inside the grid), r_term_max = 150 M;
+    (2) the 512^2 image at 45 deg, camera at r = 1000 (image rays meet no
+    fluid after their emission event, so their gradient is zero and the
+    sweep ends at the emission event)."""
+    import torch
+    import grt_amd
+    from grt_amd.fm_torus import fishbone_moncrief
+    dev = torch.device("cuda:0")
+    fields = fishbone_moncrief(nr=128, ntheta=64, nphi=64, r_min=1.5, r_max=100.0, mass=1.0,
+                               spin=0.9, r_in=6.0, r_pressure_max=12.0)
+    base = dict(a_bh=0.9, m_bh=1.0, r_term_min=1.0 + np.sqrt(1.0 - 0.81), atol=1e-8,
+                rtol=1e-8, integrator=0, absorption_enabled=1, emission_enabled=1,
+                absorption_coefficient=0.05, emission_coefficient=1.0, camera_distance=1000.0,
+                camera_theta=np.pi / 4, camera_phi=0.0, camera_width=50.0, camera_height=50.0)
+    out = {}
+    cases = (("disk", dict(base, mode=1, r_term_max=150.0, superphoton_count_normalization=
+                           args.adjoint_disk_norm, seed=11), 0.05, 40000),
+             ("image", dict(base, mode=0, r_term_max=1500.0,
+                            superphoton_count_normalization=1e8), 5e-5, 50000))
+    for name, kw, init_step, max_steps in cases:
+        ctx = grt_amd.Context(grt_amd.default_params(**kw))
+        ctx.set_grid(fields)
+        if name == "disk":
+            n = min(ctx.disk_photon_count(), args.adjoint_photons)
+            ph0 = grt_amd.empty_photons(n, device=dev)
+            ctx.init_disk(ph0, init_step)
+        else:
+            m = args.adjoint_image
+            u = -25.0 + (np.arange(m) + 0.5) * 50.0 / m
+            U, V = np.meshgrid(u, u, indexing="xy")
+            n = m * m
+            ph0 = grt_amd.empty_photons(n, device=dev)
+            ctx.init_image(ph0, init_step, torch.from_numpy(U.ravel().copy()).to(dev),
+                           torch.from_numpy(V.ravel().copy()).to(dev))
+        torch.cuda.synchronize()
+
+        def forward():
+            ph = {k: v.clone() for k, v in ph0.items()}
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st = ctx.trace(ph, max_steps)
+            torch.cuda.synchronize()
+            return time.perf_counter() - t0, st
+
+        def taped_and_adjoint():
+            ph = {k: v.clone() for k, v in ph0.items()}
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ctx.trace_taped(ph, max_steps, slots_per_photon=args.adjoint_slots)
+            I = ph["I"]
+            d = I - 0.9 * I
+            seed = d / torch.clamp(torch.linalg.vector_norm(d), min=1e-300)
+            g = ctx.adjoint(seed)
+            torch.cuda.synchronize()
+            return time.perf_counter() - t0, g
+
+        # the first call of each allocates its buffers (lists, checkpoint
+        # tape, replay records); time the second
+        forward()
+        taped_and_adjoint()
+        t_fwd, st = forward()
+        t_adj, (gr, gt, gc) = taped_and_adjoint()
+        out[name] = {
+            "photons": n,
+            "rk4_steps": st["accepted_steps"],
+            "forward_s": t_fwd,
+            "taped_forward_plus_adjoint_s": t_adj,
+            "adjoint_over_forward": t_adj / t_fwd,
+            "cells_with_gradient": int((gr != 0).sum().item()),
+            "dL_dcoeff": [float(gc[0].item()), float(gc[1].item())],
+        }
+        log(f"adjoint leg {name}: {n} photons, forward {t_fwd:.3f} s, "
+            f"taped+adjoint {t_adj:.3f} s, ratio {t_adj / t_fwd:.2f}")
+        del ph0, gr, gt
+        torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(fields, kw, su, sv, sample, threads, max_steps):
     """The oracle (FP64 C/OpenMP restatement, reference O(n) cell search) on
     the same rays: all of them when sample is 0, else an evenly strided
@@ -183,6 +267,12 @@ def main():
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the FP64-vs-mixed comparison on the 2048^2 a=0.94 image")
     ap.add_argument("--precision-image", type=int, default=2048)
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint/forward cost measurement")
+    ap.add_argument("--adjoint-photons", type=int, default=262144)
+    ap.add_argument("--adjoint-disk-norm", type=float, default=8.0)
+    ap.add_argument("--adjoint-image", type=int, default=512)
+    ap.add_argument("--adjoint-slots", type=int, default=256)
     ap.add_argument("--precision-steps", type=int, default=3)
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -305,6 +395,9 @@ def main():
         del batches
         torch.cuda.empty_cache()
         result["mixed_precision"] = precision_leg(args, log)
+
+    if not args.no_adjoint:
+        result["adjoint"] = adjoint_leg(args, log)
 
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

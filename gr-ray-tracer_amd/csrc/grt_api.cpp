@@ -55,6 +55,18 @@ struct grt_ctx {
     grt::TraceConsts h_consts{};   // host source of the copy to d_consts
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
+    // differentiable form: geodesic checkpoints of the last taped trace and
+    // the replay records of the reverse sweep
+    double* d_tape = nullptr;
+    size_t tape_cap = 0;           // doubles
+    uint64_t* d_thdr = nullptr;
+    size_t thdr_cap = 0;
+    uint64_t tape_n = 0;           // photons of the last taped trace (0: none)
+    int tape_slots = 0;
+    int tape_integrator = 0;
+    double* d_rec = nullptr;
+    uint32_t rec_lanes = 0;
+
     // device staging for the host-pointer entry points
     void* d_stage = nullptr;
     size_t stage_cap = 0;
@@ -329,7 +341,7 @@ int grt_destroy(grt_ctx* c)
     if (!c) return GRT_OK;
     free_grid(c);
     void* bufs[] = {c->d_lists, c->d_rng, c->d_counters, c->d_stats, c->d_stage, c->d_img,
-                    c->d_consts};
+                    c->d_consts, c->d_tape, c->d_thdr, c->d_rec};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -429,9 +441,13 @@ GridView make_grid_view(const grt_ctx* c)
 
 }  // namespace
 
-int grt_trace_device(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps,
-                     double termination_percent, int64_t output_interval, void* stream_v,
-                     grt_stats* stats)
+namespace {
+
+// integrate_geodesics' loop on the device; tape_slots > 0 also writes the
+// geodesic checkpoints of the differentiable form (FP64 kernel).
+int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps,
+               double termination_percent, int64_t output_interval, void* stream_v,
+               grt_stats* stats, int tape_slots)
 {
     if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
     if (!dev || !photons_complete(dev)) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
@@ -476,6 +492,30 @@ int grt_trace_device(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max
     L.use_lds = lds_doubles > 0 ? 1 : 0;
     L.lds_doubles = lds_doubles;
     L.mixed = c->params.precision == GRT_PRECISION_MIXED ? 1 : 0;
+    if (tape_slots > 0) {
+        const size_t need = (size_t)n * (size_t)tape_slots * grt::kCkptDoubles;
+        if (need > c->tape_cap) {
+            if (c->d_tape) (void)hipFree(c->d_tape);
+            c->d_tape = nullptr;
+            c->tape_cap = 0;
+            GRT_HIP(c, hipMalloc(&c->d_tape, std::max<size_t>(need, 1) * sizeof(double)));
+            c->tape_cap = need;
+        }
+        if (n > c->thdr_cap) {
+            if (c->d_thdr) (void)hipFree(c->d_thdr);
+            c->d_thdr = nullptr;
+            c->thdr_cap = 0;
+            GRT_HIP(c, hipMalloc(&c->d_thdr, std::max<uint64_t>(n, 1) * sizeof(uint64_t)));
+            c->thdr_cap = n;
+        }
+        GRT_HIP(c, hipMemsetAsync(c->d_thdr, 0, std::max<uint64_t>(n, 1) * sizeof(uint64_t), stream));
+        L.tape = c->d_tape;
+        L.tape_hdr = c->d_thdr;
+        L.tape_slots = tape_slots;
+        L.tape_overflow = c->d_counters + 3;
+        L.mixed = 0;
+        c->tape_n = 0;
+    }
     const size_t lds_bytes = (size_t)lds_doubles * sizeof(double);
     const int block = c->tune.block;
     const int persistent_blocks = std::max(1, c->num_cus * c->tune.blocks_per_cu);
@@ -531,6 +571,19 @@ int grt_trace_device(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max
         if (fraction >= termination_percent) break;
         if (!exact_schedule && live == 0) break;
     }
+    if (tape_slots > 0) {
+        GRT_HIP(c, grt::launch_tape_finalize(c->d_thdr, n, tape_slots, c->d_counters + 3, stream));
+        uint32_t overflow = 0;
+        GRT_HIP(c, hipMemcpyAsync(&overflow, c->d_counters + 3, sizeof overflow,
+                                  hipMemcpyDeviceToHost, stream));
+        GRT_HIP(c, hipStreamSynchronize(stream));
+        if (overflow)
+            return fail(c, GRT_E_RANGE, "a photon needs more than slots_per_photon geodesic "
+                                        "checkpoints; raise slots_per_photon");
+        c->tape_n = n;
+        c->tape_slots = tape_slots;
+        c->tape_integrator = P.integrator;
+    }
     unsigned long long hs[grt::kStatCount];
     GRT_HIP(c, hipMemcpy(hs, c->d_stats, sizeof hs, hipMemcpyDeviceToHost));
     if (!exact_schedule && live == 0) {
@@ -552,6 +605,129 @@ int grt_trace_device(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max
     st.kernel_ms = kernel_ms;
     if (stats) *stats = st;
     return GRT_OK;
+}
+
+}  // namespace
+
+int grt_trace_device(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps,
+                     double termination_percent, int64_t output_interval, void* stream_v,
+                     grt_stats* stats)
+{
+    return trace_impl(c, dev, n, max_steps, termination_percent, output_interval, stream_v, stats,
+                      0);
+}
+
+int grt_trace_taped_device(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps,
+                           double termination_percent, int64_t output_interval,
+                           int32_t slots_per_photon, void* stream_v, grt_stats* stats)
+{
+    if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
+    if (slots_per_photon <= 0) return fail(c, GRT_E_INVALID, "slots_per_photon must be positive");
+    if (c->params.scattering_enabled)
+        return fail(c, GRT_E_INVALID, "the differentiable form requires scattering.enabled = false");
+    return trace_impl(c, dev, n, max_steps, termination_percent, output_interval, stream_v, stats,
+                      slots_per_photon);
+}
+
+int grt_adjoint_device(grt_ctx* c, const double* dL_dI, uint64_t n, double* grad_rho,
+                       double* grad_tgas, double* grad_coeff, void* stream_v)
+{
+    if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
+    if (!dL_dI || !grad_rho || !grad_tgas || !grad_coeff)
+        return fail(c, GRT_E_INVALID, "seed and gradient arrays must be non-NULL");
+    if (!c->has_grid) return fail(c, GRT_E_NOGRID, "the adjoint needs a fluid grid (grt_set_grid)");
+    if (c->tape_n == 0 || c->tape_n != n)
+        return fail(c, GRT_E_STATE, "no taped trace of this many photons: call grt_trace_taped first");
+    if (c->params.scattering_enabled)
+        return fail(c, GRT_E_INVALID, "the differentiable form requires scattering.enabled = false");
+    if (c->params.integrator != c->tape_integrator)
+        return fail(c, GRT_E_STATE, "the integrator changed since the taped trace");
+    hipStream_t stream = static_cast<hipStream_t>(stream_v);
+    GRT_HIP(c, hipSetDevice(c->device));
+    const int block = c->tune.block;
+    const int blocks = std::max(1, c->num_cus * c->tune.blocks_per_cu);
+    const uint32_t lanes = (uint32_t)blocks * (uint32_t)block;
+    if (lanes > c->rec_lanes) {
+        if (c->d_rec) (void)hipFree(c->d_rec);
+        c->d_rec = nullptr;
+        c->rec_lanes = 0;
+        GRT_HIP(c, hipMalloc(&c->d_rec, (size_t)lanes * grt::kRecEntries * grt::kRecDoubles *
+                                            sizeof(double)));
+        c->rec_lanes = lanes;
+    }
+    // The checkpoint replay needs the launch constants of the taped trace,
+    // which the last trace left in d_consts; refresh P from the current
+    // parameters (the coefficients may have changed, the geometry may not).
+    StepParams P = c->sp;
+    P.absorb = P.absorb && c->has_grid;
+    P.emit = P.emit && c->has_grid;
+    P.scatter = 0;
+    c->h_consts.P = P;
+    GRT_HIP(c, hipMemcpyAsync(c->d_consts, &c->h_consts, sizeof c->h_consts,
+                              hipMemcpyHostToDevice, stream));
+    GRT_HIP(c, hipMemsetAsync(grad_rho, 0, c->ncell * sizeof(double), stream));
+    GRT_HIP(c, hipMemsetAsync(grad_tgas, 0, c->ncell * sizeof(double), stream));
+    GRT_HIP(c, hipMemsetAsync(grad_coeff, 0, 2 * sizeof(double), stream));
+    GRT_HIP(c, hipMemsetAsync(c->d_counters + 4, 0, sizeof(uint32_t), stream));
+    grt::AdjointLaunch A{};
+    A.consts = c->d_consts;
+    A.grid = make_grid_view(c);
+    A.integrator = P.integrator;
+    A.tape = c->d_tape;
+    A.tape_hdr = c->d_thdr;
+    A.tape_slots = c->tape_slots;
+    A.seed = dL_dI;
+    A.n = (uint32_t)n;
+    A.queue = c->d_counters + 4;
+    A.rec = c->d_rec;
+    A.rec_lanes = c->rec_lanes;
+    A.grad_rho = grad_rho;
+    A.grad_T = grad_tgas;
+    A.grad_coeff = grad_coeff;
+    A.density_scale = c->params.density_g_cm3_per_code;
+    A.temperature_scale = c->params.temperature_k_per_code;
+    const int lds_doubles = c->separable ? (int)c->table_doubles : 0;
+    A.use_lds = lds_doubles > 0 ? 1 : 0;
+    A.lds_doubles = lds_doubles;
+    GRT_HIP(c, grt::launch_adjoint(A, blocks, block, (size_t)lds_doubles * sizeof(double), stream));
+    GRT_HIP(c, hipStreamSynchronize(stream));
+    return GRT_OK;
+}
+
+int grt_set_fluid_scalars_device(grt_ctx* c, const double* rho, const double* tgas,
+                                 void* stream_v)
+{
+    if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
+    if (!rho || !tgas) return fail(c, GRT_E_INVALID, "rho and Tgas must be non-NULL");
+    if (!c->has_grid) return fail(c, GRT_E_NOGRID, "no grid: call grt_set_grid first");
+    GRT_HIP(c, hipSetDevice(c->device));
+    GRT_HIP(c, grt::launch_set_fluid_scalars(c->d_cells, c->d_rho_code, rho, tgas, c->ncell,
+                                             c->params.density_g_cm3_per_code,
+                                             c->params.temperature_k_per_code,
+                                             static_cast<hipStream_t>(stream_v)));
+    return GRT_OK;
+}
+
+int grt_set_fluid_scalars(grt_ctx* c, const double* rho, const double* tgas)
+{
+    if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
+    if (!rho || !tgas) return fail(c, GRT_E_INVALID, "rho and Tgas must be non-NULL");
+    if (!c->has_grid) return fail(c, GRT_E_NOGRID, "no grid: call grt_set_grid first");
+    GRT_HIP(c, hipSetDevice(c->device));
+    double* d = nullptr;
+    GRT_HIP(c, hipMalloc(&d, 2 * c->ncell * sizeof(double)));
+    hipError_t e = hipMemcpy(d, rho, c->ncell * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(d + c->ncell, tgas, c->ncell * sizeof(double), hipMemcpyHostToDevice);
+    int rc = GRT_OK;
+    if (e != hipSuccess) rc = hip_fail(c, e, "hipMemcpy(fluid scalars)");
+    if (rc == GRT_OK) rc = grt_set_fluid_scalars_device(c, d, d + c->ncell, nullptr);
+    if (rc == GRT_OK) {
+        e = hipDeviceSynchronize();
+        if (e != hipSuccess) rc = hip_fail(c, e, "set_fluid_scalars");
+    }
+    (void)hipFree(d);
+    return rc;
 }
 
 int grt_trace(grt_ctx* c, const grt_photons* host, uint64_t n, int32_t max_steps,

@@ -133,7 +133,6 @@ GRT_HD void ks_rhs(const double s[8], double a, double m, double out[8])
     const double r2 = r * r;
     const double r3 = r2 * r;
     const double D = r3 * r + az * az;
-    const double D2 = D * D;
 #if GRT_RHS_RECIPROCALS
     // (mixed-precision form) Four reciprocals replace the 21 divisions and
     // each quotient is a plain product with one of them, within an ulp or
@@ -151,6 +150,7 @@ GRT_HD void ks_rhs(const double s[8], double a, double m, double out[8])
     // from the correctly rounded reciprocal y = 1/d of its divisor with
     // Markstein's correction (div_rn), so the five divisors cost one
     // division each instead of one per quotient.  1/(2r) = (1/r)/2 exactly.
+    const double D2 = D * D;
     const double iS = 1.0 / S, ir = 1.0 / r, iB = 1.0 / B, iD = 1.0 / D, iD2 = 1.0 / D2;
     const double itr = 0.5 * ir;
     const double f = 1.0 + div_rn(A, S, iS);

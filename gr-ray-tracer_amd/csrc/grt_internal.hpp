@@ -60,6 +60,33 @@ struct TraceLaunch {
     int use_lds;              // stage the separable search tables in LDS
     int lds_doubles;          // size of that staging area
     int mixed;                // GRT_PRECISION_MIXED: run the mixed FP32/FP64 kernel
+    // differentiable form: geodesic checkpoints (tape != nullptr selects the
+    // checkpointing kernel)
+    double* tape;             // [photon][tape_slots][kCkptDoubles]
+    uint64_t* tape_hdr;       // per photon: calls | committed << 32 | touched << 63
+    int tape_slots;
+    uint32_t* tape_overflow;  // set when a photon needs more than tape_slots
+};
+
+// Reverse sweep of the differentiable form (grt_adjoint.hip).
+struct AdjointLaunch {
+    const TraceConsts* consts;
+    GridView grid;
+    int integrator;
+    const double* tape;
+    const uint64_t* tape_hdr;
+    int tape_slots;
+    const double* seed;       // dL/dI of every photon's final Stokes I
+    uint32_t n;
+    uint32_t* queue;          // zeroed before launch
+    double* rec;              // per-lane replay records, interleaved by lane
+    uint32_t rec_lanes;       // lanes the record buffer holds
+    double* grad_rho;         // per cell, code units
+    double* grad_T;
+    double* grad_coeff;       // [absorption, emission]
+    double density_scale, temperature_scale;
+    int use_lds;
+    int lds_doubles;
 };
 
 // Kernel launchers (grt_kernels.hip).  They only enqueue work on `stream`.
@@ -68,6 +95,20 @@ hipError_t launch_trace(const TraceLaunch& a, int blocks, int block, size_t lds_
 // grt_mixed.hip
 hipError_t launch_trace_mixed(const TraceLaunch& a, int blocks, int block, size_t lds_bytes,
                               hipStream_t stream);
+// grt_adjoint.hip
+hipError_t launch_trace_taped(const TraceLaunch& a, int blocks, int block, size_t lds_bytes,
+                              hipStream_t stream);
+hipError_t launch_tape_finalize(uint64_t* hdr, uint64_t n, int slots, uint32_t* overflow,
+                                hipStream_t stream);
+hipError_t launch_adjoint(const AdjointLaunch& a, int blocks, int block, size_t lds_bytes,
+                          hipStream_t stream);
+hipError_t launch_set_fluid_scalars(double* cells, double* rho_code, const double* rho,
+                                    const double* tgas, uint64_t ncell, double density_scale,
+                                    double temperature_scale, hipStream_t stream);
+constexpr uint32_t kSegmentCalls = 32;   // functor calls per geodesic checkpoint
+constexpr int kCkptDoubles = 20;   // s[8], stokes[4], freq, eem, dl, phase, draw, call0, id, pad
+constexpr int kRecEntries = 64;    // replay records per lane: <= 2 per call
+constexpr int kRecDoubles = 16;    // cell (or -1: reset), I before, step, frequency, fluid[12]
 hipError_t launch_build_live(const uint8_t* term, uint64_t n, uint32_t* list, uint32_t* count,
                              hipStream_t stream);
 hipError_t launch_image_products(const PhotonPtrs& ph, uint64_t n, int64_t nx, int64_t ny,

@@ -171,6 +171,7 @@ hipError_t launch_init_image(const PhotonPtrs& ph, uint64_t n, uint64_t id0, con
 hipError_t launch_trace(const TraceLaunch& a, int blocks, int block, size_t lds_bytes,
                         hipStream_t stream)
 {
+    if (a.tape) return launch_trace_taped(a, blocks, block, lds_bytes, stream);
     if (a.mixed) return launch_trace_mixed(a, blocks, block, lds_bytes, stream);
     const bool sep = a.grid.separable || !a.grid.available;
     if (a.integrator == GRT_RK4) {

@@ -88,9 +88,81 @@ __device__ __forceinline__ void store_photon(const PhotonPtrs& p, uint32_t idx, 
 #define GRT_TRACE_WAVES_PER_EU 2
 #endif
 
+// ---------------------------------------------------------------------------
+// Geodesic checkpoints of the differentiable form.
+//
+// A photon's functor calls are cut into segments of kSegmentCalls.  At the
+// start of every segment the photon's state is written to checkpoint slot
+// `committed`; the slot is kept (committed + 1) only if something in the
+// segment touched Stokes I (a radiative-transfer update or an event that sets
+// I outright).  Per photon a 64-bit header holds the calls made so far (low
+// 32 bits), the committed count (bits 32-62) and the "segment touched I" flag
+// (bit 63).  The adjoint replays each kept segment from its checkpoint.
+// ---------------------------------------------------------------------------
+
+constexpr uint64_t kTapeActive = 1ull << 63;
+
+struct TapeFlag {
+    bool touched = false;
+    GRT_HD void rt(const FluidCellCGS&, uint32_t, double, double, double) { touched = true; }
+    GRT_HD void reset() { touched = true; }
+};
+
+__device__ __forceinline__ void write_checkpoint(double* c, const Photon& ph, const PhotonRng& rng,
+                                                 uint32_t call0)
+{
+#pragma unroll
+    for (int q = 0; q < 8; ++q) c[q] = ph.s[q];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) c[8 + q] = ph.stokes[q];
+    c[12] = ph.freq;
+    c[13] = ph.eem;
+    c[14] = ph.dl;
+    c[15] = (double)ph.phase;
+    c[16] = (double)rng.draw;
+    c[17] = (double)call0;
+    c[18] = __longlong_as_double((long long)rng.id);
+    c[19] = 0.0;
+}
+
+__device__ __forceinline__ void read_checkpoint(const double* c, Photon& ph, PhotonRng& rng,
+                                                uint32_t& call0)
+{
+#pragma unroll
+    for (int q = 0; q < 8; ++q) ph.s[q] = c[q];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) ph.stokes[q] = c[8 + q];
+    ph.freq = c[12];
+    ph.eem = c[13];
+    ph.dl = c[14];
+    ph.phase = (uint8_t)c[15];
+    ph.term = 0;
+    rng.draw = (uint32_t)c[16];
+    call0 = (uint32_t)c[17];
+    rng.id = (uint64_t)__double_as_longlong(c[18]);
+}
+
+// Header update at the end of a functor call; returns the new header.
+__device__ __forceinline__ uint64_t tape_end_call(uint64_t h, bool touched, bool term, int slots,
+                                                  uint32_t* overflow)
+{
+    const uint32_t calls = (uint32_t)h + 1u;
+    uint32_t committed = (uint32_t)(h >> 32) & 0x7FFFFFFFu;
+    bool active = (h & kTapeActive) != 0 || touched;
+    if ((calls % kSegmentCalls) == 0 || term) {
+        if (active) {
+            if (committed < (uint32_t)slots) ++committed;
+            else atomicOr(overflow, 1u);
+        }
+        active = false;
+    }
+    return ((uint64_t)committed << 32) | calls | (active ? kTapeActive : 0ull);
+}
+
 // The persistent loop shared by the tracing kernels; Rhs and kMixed select
-// the arithmetic of the step (see lane_iterate).
-template <int kIntegrator, int kSep, class Rhs, int kMixed>
+// the arithmetic of the step (see lane_iterate), kTape the geodesic
+// checkpoints of the differentiable form.
+template <int kIntegrator, int kSep, class Rhs, int kMixed, int kTape = 0>
 __device__ __forceinline__ void trace_body(const TraceLaunch& a)
 {
     extern __shared__ double lds[];
@@ -114,20 +186,39 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a)
     uint32_t slot = wave_claim(a.queue);
     bool have = slot < a.n_in;
     uint32_t idx = have ? a.list_in[slot] : 0u;
-    if (have) load_photon(PH, idx, L.ph, rng);
+    uint64_t thdr = 0;
+    if (have) {
+        load_photon(PH, idx, L.ph, rng);
+        if (kTape) thdr = a.tape_hdr[idx];
+    }
     lane_reset(L);
     int calls_left = a.budget;
     int calls_done = 0;
 
     while (have) {
         double au = 0.0, av = 0.0;
-        const bool call_done = lane_iterate<kIntegrator, kSep, Rhs, kMixed>(L, G, P, rng, n, au, av);
+        if (kTape && !L.in_call && ((uint32_t)thdr % kSegmentCalls) == 0) {
+            const uint32_t k = (uint32_t)(thdr >> 32) & 0x7FFFFFFFu;
+            if (k < (uint32_t)a.tape_slots)
+                write_checkpoint(a.tape + ((size_t)idx * a.tape_slots + k) * kCkptDoubles, L.ph,
+                                 rng, (uint32_t)thdr);
+        }
+        TapeFlag flag;
+        bool call_done;
+        if (kTape)
+            call_done = lane_iterate<kIntegrator, kSep, Rhs, kMixed, TapeFlag>(L, G, P, rng, n,
+                                                                              au, av, flag);
+        else
+            call_done = lane_iterate<kIntegrator, kSep, Rhs, kMixed>(L, G, P, rng, n, au, av);
+        if (kTape && flag.touched) thdr |= kTapeActive;
         if (call_done) {
             ++n.calls;
             ++calls_done;
             --calls_left;
+            if (kTape) thdr = tape_end_call(thdr, false, L.ph.term != 0, a.tape_slots, a.tape_overflow);
             if (L.ph.term || calls_left == 0) {
                 store_photon(PH, idx, L.ph, rng, au, av);
+                if (kTape) a.tape_hdr[idx] = thdr;
                 if (L.ph.term) {
                     const uint32_t step = (uint32_t)(a.step0 + calls_done - 1);
                     last_term = step > last_term ? step : last_term;
@@ -140,6 +231,7 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a)
                 if (have) {
                     idx = a.list_in[slot];
                     load_photon(PH, idx, L.ph, rng);
+                    if (kTape) thdr = a.tape_hdr[idx];
                     lane_reset(L);
                     calls_left = a.budget;
                     calls_done = 0;

@@ -86,7 +86,8 @@ EXPORTED = [
     "grt_params_default", "grt_create", "grt_destroy", "grt_set_params", "grt_set_grid",
     "grt_trace", "grt_trace_device", "grt_image_products", "grt_image_products_device",
     "grt_init_image", "grt_init_image_device", "grt_disk_photon_count", "grt_init_disk",
-    "grt_init_disk_device",
+    "grt_init_disk_device", "grt_trace_taped_device", "grt_adjoint_device",
+    "grt_set_fluid_scalars", "grt_set_fluid_scalars_device",
     "grt_get_tuning", "grt_set_tuning", "grt_last_error", "grt_abi_version",
 ]
 
@@ -124,6 +125,11 @@ def load():
         "grt_disk_photon_count": (C.c_int, [vp, C.POINTER(C.c_uint64)]),
         "grt_init_disk": (C.c_int, [vp, PH, C.c_uint64, C.c_uint64, C.c_double]),
         "grt_init_disk_device": (C.c_int, [vp, PH, C.c_uint64, C.c_uint64, C.c_double, vp]),
+        "grt_trace_taped_device": (C.c_int, [vp, PH, C.c_uint64, C.c_int32, C.c_double,
+                                             C.c_int64, C.c_int32, vp, ST]),
+        "grt_adjoint_device": (C.c_int, [vp, vp, C.c_uint64, vp, vp, vp, vp]),
+        "grt_set_fluid_scalars": (C.c_int, [vp, vp, vp]),
+        "grt_set_fluid_scalars_device": (C.c_int, [vp, vp, vp, vp]),
         "grt_get_tuning": (C.c_int, [vp, TU]),
         "grt_set_tuning": (C.c_int, [vp, TU]),
         "grt_last_error": (C.c_char_p, [vp]),
@@ -195,6 +201,7 @@ class Context:
         self._h = h
         self.params = params
         self._keep = None
+        self._grid_shape = None
 
     def _check(self, rc):
         if rc != 0:
@@ -229,6 +236,7 @@ class Context:
         self._check(self._L.grt_set_grid(
             self._h, nr, nth, nph, *[arr[k].ctypes.data for k in
                                      ("r", "theta", "phi", "rho", "Tgas", "vel", "mag")]))
+        self._grid_shape = (nr, nth, nph)
 
     def trace(self, photons, max_steps, termination_percent=1.0, output_interval=0,
               stream=None):
@@ -247,6 +255,53 @@ class Context:
             self._check(self._L.grt_trace(self._h, C.byref(s), n, max_steps,
                                           termination_percent, output_interval, C.byref(st)))
         return _stats_dict(st)
+
+    def trace_taped(self, photons, max_steps, slots_per_photon=64, termination_percent=1.0,
+                    output_interval=0, stream=None):
+        """trace() with the FP64 kernel, keeping the geodesic checkpoints the
+        adjoint replays (photons: dict of GPU tensors)."""
+        n = len(photons["x0"])
+        st = Stats()
+        s = photon_struct(photons)
+        self._check(self._L.grt_trace_taped_device(self._h, C.byref(s), n, max_steps,
+                                                   termination_percent, output_interval,
+                                                   slots_per_photon, _stream_handle(stream),
+                                                   C.byref(st)))
+        self._taped_n = n
+        return _stats_dict(st)
+
+    def adjoint(self, dL_dI, stream=None):
+        """Reverse sweep of the last taped trace.  dL_dI: float64 GPU tensor,
+        one entry per photon.  Returns (dL/drho, dL/dTgas) shaped like the
+        grid and dL/d[absorption_coefficient, emission_coefficient]."""
+        import torch
+        dL_dI = dL_dI.to(torch.float64).contiguous()
+        dev = dL_dI.device
+        gr = torch.empty(self._grid_shape, dtype=torch.float64, device=dev)
+        gt = torch.empty(self._grid_shape, dtype=torch.float64, device=dev)
+        gc = torch.empty(2, dtype=torch.float64, device=dev)
+        self._check(self._L.grt_adjoint_device(self._h, dL_dI.data_ptr(), dL_dI.numel(),
+                                               gr.data_ptr(), gt.data_ptr(), gc.data_ptr(),
+                                               _stream_handle(stream)))
+        return gr, gt, gc
+
+    def set_fluid_scalars(self, rho, tgas, stream=None):
+        """Replace every cell's density and temperature (numpy arrays or GPU
+        tensors shaped like the grid), keeping the rest of the grid."""
+        if hasattr(rho, "data_ptr"):
+            import torch
+            r = rho.to(torch.float64).contiguous()
+            t = tgas.to(torch.float64).contiguous()
+            if tuple(r.shape) != self._grid_shape or tuple(t.shape) != self._grid_shape:
+                raise GrtError("rho and Tgas must have the grid's shape")
+            self._check(self._L.grt_set_fluid_scalars_device(self._h, r.data_ptr(), t.data_ptr(),
+                                                             _stream_handle(stream)))
+        else:
+            r = np.ascontiguousarray(rho, dtype=np.float64)
+            t = np.ascontiguousarray(tgas, dtype=np.float64)
+            if r.shape != self._grid_shape or t.shape != self._grid_shape:
+                raise GrtError("rho and Tgas must have the grid's shape")
+            self._check(self._L.grt_set_fluid_scalars(self._h, r.ctypes.data, t.ctypes.data))
 
     def init_image(self, photons, initial_step, screen_u=None, screen_v=None, id_offset=0,
                    stream=None):

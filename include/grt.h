@@ -27,7 +27,9 @@ enum {
     GRT_E_INVALID = -1,   /* bad argument, shape or configuration */
     GRT_E_NOGRID = -2,    /* a fluid-dependent feature with no grid set */
     GRT_E_DEVICE = -3,    /* HIP runtime error */
-    GRT_E_NOMEM = -4      /* device allocation failed */
+    GRT_E_NOMEM = -4,     /* device allocation failed */
+    GRT_E_STATE = -5,     /* call out of sequence (e.g. adjoint without a taped trace) */
+    GRT_E_RANGE = -6      /* a capacity given by the caller was exceeded */
 };
 
 enum { GRT_RK4 = 0, GRT_RK45 = 1 };                 /* utils.hpp:169 IntegratorType */
@@ -199,6 +201,45 @@ int grt_image_products_device(grt_ctx *ctx, const grt_photons *dev, uint64_t n, 
                               int64_t ny, int64_t spectrum_bins, double spectrum_min_hz,
                               double spectrum_max_hz, double *image, double *spectrum_nu,
                               double *spectrum_I, uint64_t *arrived, void *stream);
+
+/*
+ * The differentiable form (no counterpart in the reference, whose fluid
+ * parameters enter only through the radiative-transfer update
+ * apply_radiative_transfer_step, radiative_transfer.hpp:81-96).
+ *
+ * grt_trace_taped_device traces like grt_trace_device, always with the FP64
+ * kernel, and keeps geodesic checkpoints: one per 32 functor calls of a
+ * photon whose 32 calls touch Stokes I (a transfer update, the image-mode
+ * emission event, or a zeroing), at most slots_per_photon per photon
+ * (GRT_E_RANGE beyond that).  Scattering must be disabled.
+ *
+ * grt_adjoint_device then returns, for a loss L whose gradient with respect
+ * to each photon's final Stokes I is dL_dI[n] (device array, the photons of
+ * the taped trace in the same order), dL/drho and dL/dTgas per cell (device
+ * arrays of nr*nth*nph doubles, code units, overwritten) and
+ * grad_coeff[0..1] = dL/d absorption_coefficient, dL/d emission_coefficient
+ * (device array, overwritten).  It replays each kept segment from its
+ * checkpoint and sweeps the transfer recurrence backwards; the partial
+ * derivatives of the increments come from the ...Gradient companions of the
+ * extension points.  The coefficients may change between the taped trace
+ * and the adjoint (the adjoint then differentiates at the new values along
+ * the same geodesics); the geometry and the grid may not.  The image-mode
+ * emission energy E / int(rho * norm) is piecewise constant in rho and is
+ * treated as a constant.
+ *
+ * grt_set_fluid_scalars(_device) replaces the density and temperature of
+ * every cell of the current grid (nr*nth*nph doubles each, host or device
+ * arrays) without re-preparing the rest of the grid, for optimisation loops
+ * and finite differences.
+ */
+int grt_trace_taped_device(grt_ctx *ctx, const grt_photons *dev, uint64_t n, int32_t max_steps,
+                           double termination_percent, int64_t output_interval,
+                           int32_t slots_per_photon, void *stream, grt_stats *stats);
+int grt_adjoint_device(grt_ctx *ctx, const double *dL_dI, uint64_t n, double *grad_rho,
+                       double *grad_tgas, double *grad_coeff, void *stream);
+int grt_set_fluid_scalars(grt_ctx *ctx, const double *rho, const double *tgas);
+int grt_set_fluid_scalars_device(grt_ctx *ctx, const double *rho, const double *tgas,
+                                 void *stream);
 
 /* Launch-shape control of the tracing kernel (no reference counterpart). */
 int grt_get_tuning(grt_ctx *ctx, grt_tuning *t);

@@ -5,8 +5,8 @@ density floor so the torus and its shadow are visible) on a 64 x 32 x 32
 grid, camera at 17 deg, RK45 rtol = atol = 1e-7.  Criteria: median per-pixel
 relative Stokes-I error over the lit pixels <= 1e-4 (the stated bound of the
 mixed form), the unlit central region (when there is one) within two pixels,
-the photons still null to 1e-9, and the mixed kernel really ran (its states
-differ from the FP64 ones in the last digits).
+the photons as close to null as in the FP64 form, and the mixed kernel
+really ran (its states differ from the FP64 ones in the last digits).
 
 This torus reaches the grid edge and fills 0.30 < theta < 2.84, so at 17 deg
 the centre of the image sees torus surface and no shadow; the ring diameter
@@ -70,12 +70,22 @@ def test_mixed_image_accuracy(gpu):
         assert abs(dm - d64) <= 2 * 50.0 / 128
     # both forms take comparable numbers of steps
     assert abs(stm["accepted_steps"] / st64["accepted_steps"] - 1.0) < 0.05
-    # the mixed kernel ran: final states agree closely but not bit for bit
+    # the mixed kernel ran: results differ in the last digits, and the arrival
+    # points on the screen agree far below a pixel (the final position itself
+    # includes the overshoot past the camera plane, which moves with the
+    # last step's length)
     ok = (ph64["phase"] == grt_amd.PHASE_ARRIVED) & (phm["phase"] == grt_amd.PHASE_ARRIVED)
-    dx = np.abs(phm["x1"][ok] - ph64["x1"][ok])
-    assert dx.max() > 0.0 and np.median(dx) < 1e-3
-    # photons stay on the null cone
-    assert np.nanmax(null_norm(phm, kw["a_bh"], 1.0)[ok]) < 1e-9
+    assert np.abs(phm["x1"][ok] - ph64["x1"][ok]).max() > 0.0
+    du = np.abs(phm["theta_disp"][ok] - ph64["theta_disp"][ok])
+    dv = np.abs(phm["phi_disp"][ok] - ph64["phi_disp"][ok])
+    assert np.median(du) < 1e-6 and np.median(dv) < 1e-6
+    assert np.mean((du < 1e-3) & (dv < 1e-3)) > 0.99
+    # photons stay on the null cone as well as in the FP64 form (the RK45
+    # truncation error at rtol 1e-7 sets both)
+    nm = np.nanmax(null_norm(phm, kw["a_bh"], 1.0)[ok])
+    nf = np.nanmax(null_norm(ph64, kw["a_bh"], 1.0)[ok])
+    print("null norm max: mixed %.3e fp64 %.3e" % (nm, nf))
+    assert nm <= 10.0 * nf + 1e-12
 
 
 def test_mixed_rk4_and_general_grid(gpu):
