@@ -119,9 +119,13 @@ def precision_leg(args, log):
         ctx.set_grid(fields)
         proto = grt_amd.empty_photons(n * n, device=dev)
         ctx.init_image(proto, 0.1, su, sv)
+        if args.autotune:
+            from grt_amd.autotune import autotune
+            autotune(ctx, proto, args.max_steps, repeats=1)
         wall, tot, last = timed_renders(ctx, proto, args.precision_steps, 1, args.max_steps)
         img = ctx.image_products(last, n, n)
-        out[name] = dict(wall=wall, totals=tot, image=img["image"][0].cpu().numpy())
+        out[name] = dict(wall=wall, totals=tot, image=img["image"][0].cpu().numpy(),
+                         tuning=ctx.tuning)
         log(f"precision form {name}: {args.precision_steps} renders in {wall * 1e3:.1f} ms, "
             f"{tot['accepted_steps'] / wall:.3e} steps/s")
         del proto, last, ctx
@@ -144,6 +148,7 @@ def precision_leg(args, log):
                                       "mixed": m["totals"]["accepted_steps"] / args.precision_steps},
         "stokes_I_vs_fp64": err,
         "ring_diameter_M": {"fp64": d_f, "mixed": d_m, "abs_err": abs(d_m - d_f)},
+        "tuning": {"fp64": f["tuning"], "mixed": m["tuning"]},
     }
 
 
@@ -264,6 +269,8 @@ def main():
                     help="rays traced by the CPU baseline (0: the whole image)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default launch shape instead of autotuning it first")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the FP64-vs-mixed comparison on the 2048^2 a=0.94 image")
     ap.add_argument("--precision-image", type=int, default=2048)
@@ -324,6 +331,13 @@ def main():
     torch.cuda.synchronize()
     log(f"{len(batches)} batches of {n} rays resident; setup {time.perf_counter() - t_setup:.1f}s")
 
+    tuning_table = None
+    if args.autotune:
+        from grt_amd.autotune import autotune
+        best, tuning_table = autotune(ctx, proto, args.max_steps)
+        log(f"autotuned launch shape {best}: " +
+            ", ".join(f"{t['block']}x{t['blocks_per_cu']}/{t['sub_budget']}={ms:.2f}ms"
+                      for t, ms in tuning_table[:5]))
     for w in range(args.warmup):
         st = ctx.trace(batches[w], args.max_steps)
         log(f"warmup {w}: {st['accepted_steps']} steps, {st['kernel_ms']:.2f} ms kernel")
@@ -373,10 +387,13 @@ def main():
             "image": [args.image, args.image],
             "rays_per_step": n,
             "precision": "fp64",
+            "tuning": ctx.tuning,
         },
         "pixels_per_s": n * args.steps / wall,
         "requested_steps_per_s": n * totals["global_steps"] / wall,
         "per_render": {k: v / args.steps for k, v in totals.items()},
+        "autotune": ([{"tuning": t, "ms": ms} for t, ms in tuning_table[:8]]
+                     if tuning_table else None),
         "arrived_count": arrived,
         "roofline": {
             "bound": "fp64-valu",

@@ -433,7 +433,8 @@ GRT_HD double pseudo_angle(double X, double Y)
     return 3.0 + X / (X - Y);
 }
 
-// Separable fast path of locate_structured_grid_cell (scattering.hpp:151-180).
+// Separable fast path of locate_structured_grid_cell (scattering.hpp:151-180);
+// (i, j, k) are written only when the point lies in a cell.
 // With r(i,j,k) = r(i), theta(i,j,k) = theta(j), phi(i,j,k) = phi(k) the three
 // coordinate-descent passes converge in the first, and each nearest_* scan
 // (first strict minimum) reduces to a bracket search plus the candidates
@@ -460,7 +461,10 @@ GRT_HD bool locate_separable(const GridView& G, double radius, double ct, double
     const double* R = G.tab;
     const double* T = R + 4 * nr;
     const double* F = T + 4 * nth;
-    // radial (scattering.hpp:47-65)
+    // radial (scattering.hpp:47-65).  Beyond the outer window of the last
+    // cell or below the inner window of the first the nearest cell is that
+    // end cell and its window test fails, so no search is needed.
+    if (!(radius <= R[4 * nr - 1]) || !(radius >= R[2 * nr])) return false;
     int i;
     {
         const int lo = count_le(R, nr, radius) - 1;
@@ -468,6 +472,9 @@ GRT_HD bool locate_separable(const GridView& G, double radius, double ct, double
         else if (lo >= nr - 1) i = nr - 1;
         else i = (radius > R[nr + lo]) ? lo + 1 : lo;
     }
+    // within_* tests (scattering.hpp:99-149), each as soon as its index is
+    // known: the cell is only used when all three hold
+    if (!(R[2 * nr + i] <= radius && radius <= R[3 * nr + i])) return false;
     // polar (scattering.hpp:67-81)
     int j;
     {
@@ -476,6 +483,7 @@ GRT_HD bool locate_separable(const GridView& G, double radius, double ct, double
         else if (lo >= nth - 1) j = nth - 1;
         else j = (ct < T[nth + lo]) ? lo + 1 : lo;
     }
+    if (!(T[2 * nth + j] <= ct && ct <= T[3 * nth + j])) return false;
     // azimuth (scattering.hpp:83-97): candidates 0, lo, lo+1, nph-1
     int k = 0;
     double best = X * F[nph] + Y * F[2 * nph];
@@ -498,20 +506,12 @@ GRT_HD bool locate_separable(const GridView& G, double radius, double ct, double
     i_out = i;
     j_out = j;
     k_out = k;
-    // within_* tests (scattering.hpp:99-149)
-    const bool wi = R[2 * nr + i] <= radius && radius <= R[3 * nr + i];
-    const bool wj = T[2 * nth + j] <= ct && ct <= T[3 * nth + j];
     const double norm = sqrt(X * X + Y * Y);
     const double cthr = F[3 * nph + k];
-    bool wk;
-    if (cthr >= -1.0) {
-        wk = best >= cthr * norm;
-    } else {
-        // single azimuth: |sin d| <= 1e-10 on the near side
-        const double cross = X * F[2 * nph + k] - Y * F[nph + k];
-        wk = best >= 0.0 && fabs(cross) <= 1.0e-10 * norm;
-    }
-    return wi && wj && wk;
+    if (cthr >= -1.0) return best >= cthr * norm;
+    // single azimuth: |sin d| <= 1e-10 on the near side
+    const double cross = X * F[2 * nph + k] - Y * F[nph + k];
+    return best >= 0.0 && fabs(cross) <= 1.0e-10 * norm;
 }
 
 GRT_HD double c3at(const double* a, const GridView& G, int i, int j, int k)

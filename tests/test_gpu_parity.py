@@ -246,3 +246,29 @@ def test_full_size_invariants(gpu):
     img = ctx.image_products(dev, 1024, 1024)
     assert img["arrived_count"] == len(su)
     assert np.isfinite(img["image"].cpu().numpy()).all()
+
+
+def test_autotune_keeps_results(oracle, gpu, c1):
+    """Every launch shape the autotuner tries gives the same photons."""
+    import grt_amd
+    from grt_amd.autotune import autotune
+    kw = c1_params()
+    su, sv = pixel_screen(48)
+    ctx = gpu_ctx(kw, c1)
+    po, _ = oracle_parts(oracle, kw, None)
+    g0 = grt_amd.to_device(oracle.init_image(po, su, sv, 0.1))
+    best, table = autotune(ctx, g0, 10000, shapes=((256, 2), (64, 8)), budgets=(8, 1 << 20),
+                           repeats=1)
+    assert len(table) == 4 and ctx.tuning == best
+    ref = None
+    for tune, _ in table:
+        ctx.tuning = tune
+        g = {k: v.clone() for k, v in g0.items()}
+        st = ctx.trace(g, 10000)
+        h = grt_amd.to_host(g)
+        if ref is None:
+            ref = h
+        for k in ("x1", "k1", "I", "phase", "theta_disp"):
+            assert np.array_equal(h[k], ref[k]), (tune, k)
+        if tune["sub_budget"] == 8:
+            assert st["launches"] > 1
