@@ -75,8 +75,19 @@ struct grt_ctx {
 };
 
 namespace {
-
 std::string g_create_error;
+}  // namespace
+
+// Message slots shared with grt_io.cpp.
+void grt_set_global_error(const std::string& msg) { g_create_error = msg; }
+int grt_set_error(grt_ctx* ctx, int code, const std::string& msg)
+{
+    if (ctx) ctx->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+namespace {
 
 int fail(grt_ctx* ctx, int code, const std::string& msg)
 {

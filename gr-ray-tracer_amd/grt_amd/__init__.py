@@ -88,6 +88,8 @@ EXPORTED = [
     "grt_init_image", "grt_init_image_device", "grt_disk_photon_count", "grt_init_disk",
     "grt_init_disk_device", "grt_trace_taped_device", "grt_adjoint_device",
     "grt_set_fluid_scalars", "grt_set_fluid_scalars_device",
+    "grt_load_config", "grt_read_npy", "grt_load_numpy_fields", "grt_write_image_products_npz",
+    "grt_write_photons_npz",
     "grt_get_tuning", "grt_set_tuning", "grt_last_error", "grt_abi_version",
 ]
 

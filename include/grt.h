@@ -241,6 +241,63 @@ int grt_set_fluid_scalars(grt_ctx *ctx, const double *rho, const double *tgas);
 int grt_set_fluid_scalars_device(grt_ctx *ctx, const double *rho, const double *tgas,
                                  void *stream);
 
+/*
+ * The file formats either side of the path.
+ *
+ * grt_load_config replaces toml_config::load_and_apply_toml_config
+ * (toml_config.hpp:129-277): the same flat TOML subset, key set, defaults
+ * (utils.cpp:4-37, simulation_options.hpp:13-32, photon_generation.hpp:40-50)
+ * and error messages (available from grt_last_error(NULL)).  The physics,
+ * camera, integration-tolerance and unit keys fill *params (starting from
+ * grt_params_default); the run keys fill *run.  scattering.optical_depth is
+ * not read, as in the reference.
+ *
+ * grt_read_npy reads an f8 C-order .npy file (cnpy.h:302-349; the messages of
+ * load_python_arrays.hpp:82-112): with dst == NULL it only returns the shape.
+ * grt_load_numpy_fields replaces discover_numpy_field_paths +
+ * load_numpy_field_bundle (load_python_arrays.hpp:15-223): the seven files
+ * r, theta, phi, rho, Tgas (nr, ntheta, nphi) and vel, mag (.., 4) of
+ * `directory`, checked, then handed to grt_set_grid (with ctx == NULL the
+ * bundle is only checked; messages then come from grt_last_error(NULL)).
+ *
+ * grt_write_image_products_npz writes <path> with the members of
+ * write_image_products (write_output.hpp:107-142): image_I/Q/U/V (ny, nx),
+ * spectrum_frequency_hz, spectrum_I (bins) and arrived_count (1, u8);
+ * image is the [4][ny][nx] array of grt_image_products.
+ * grt_write_photons_npz writes the selected photon arrays of host SoA `ph`,
+ * every `stride`-th photon, as write_output_step does (write_output.hpp:
+ * 26-60, 90-103); variables is an output.variables string (NULL: the
+ * default selection).  Both write "<path>.tmp" and rename it over <path>
+ * (write_output.hpp:66-86), as uncompressed, non-ZIP64 archives (cnpy.h).
+ */
+typedef struct {
+    char name[256];               /* simulation.name */
+    char numpy_dir[1024];         /* input.numpy_dir */
+    char output_dir[1024];        /* output.directory */
+    char output_variables[512];   /* output.variables, normalised comma list */
+    char decomposition[128];      /* parallel.decomposition (one GPU: unused) */
+    int32_t vacuum;               /* input.vacuum */
+    int32_t max_steps;            /* integration.max_steps */
+    uint64_t photon_count;        /* photons.count */
+    double termination_percent;   /* integration.termination_percent */
+    double initial_step;          /* integration.initial_step */
+    uint64_t output_interval;     /* output.interval */
+    uint64_t output_stride;       /* output.stride */
+    uint64_t image_nx, image_ny;  /* output.image_nx / image_ny */
+    uint64_t spectrum_bins;       /* output.spectrum_bins */
+    double spectrum_min_hz, spectrum_max_hz;
+} grt_run_config;
+
+int grt_load_config(const char *path, grt_params *params, grt_run_config *run);
+int grt_read_npy(const char *path, double *dst, uint64_t capacity, uint64_t shape[4],
+                 int32_t *ndim);
+int grt_load_numpy_fields(grt_ctx *ctx, const char *directory);
+int grt_write_image_products_npz(const char *path, int64_t nx, int64_t ny, int64_t bins,
+                                 const double *image, const double *spectrum_nu,
+                                 const double *spectrum_I, uint64_t arrived);
+int grt_write_photons_npz(const char *path, const grt_photons *ph, uint64_t n,
+                          const char *variables, uint64_t stride);
+
 /* Launch-shape control of the tracing kernel (no reference counterpart). */
 int grt_get_tuning(grt_ctx *ctx, grt_tuning *t);
 int grt_set_tuning(grt_ctx *ctx, const grt_tuning *t);
