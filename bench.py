@@ -236,6 +236,57 @@ def adjoint_leg(args, log):
     return out
 
 
+def scattering_leg(args, log):
+    """Batched scattering Monte Carlo in disk mode: torus a = 0.5 (r_in = 6,
+    r_pmax = 12) on a 256 x 128 x 256 grid, about args.scatter_packets packets
+    emitted from the torus cells (superphoton normalisation chosen for that
+    count), isotropic elastic scattering in the fluid tetrad with albedo 1,
+    absorption and emission on, Philox seed 1234, RK45 rtol 1e-7, mixed
+    precision; packets escape at r = 1000 M."""
+    import torch
+    import grt_amd
+    from grt_amd.fm_torus import fishbone_moncrief
+    dev = torch.device("cuda:0")
+    f = fishbone_moncrief(nr=256, ntheta=128, nphi=256, r_min=2.0, r_max=100.0, mass=1.0,
+                          spin=0.5, r_in=6.0, r_pressure_max=12.0)
+    rho_sum = float(f["rho"].sum())
+    norm = args.scatter_packets / rho_sum
+    out = {}
+    for name, prec in (("mixed", grt_amd.PRECISION_MIXED), ("fp64", grt_amd.PRECISION_FP64)):
+        kw = dict(a_bh=0.5, m_bh=1.0, r_term_min=1.0 + np.sqrt(0.75), r_term_max=1000.0,
+                  atol=1e-7, rtol=1e-7, integrator=1, mode=1, absorption_enabled=1,
+                  emission_enabled=1, absorption_coefficient=1.0, emission_coefficient=1.0,
+                  scattering_enabled=1, albedo=1.0, seed=1234,
+                  superphoton_count_normalization=norm, density_g_cm3_per_code=0.125,
+                  precision=prec)
+        ctx = grt_amd.Context(grt_amd.default_params(**kw))
+        ctx.set_grid(f)
+        n = ctx.disk_photon_count()
+        ph = grt_amd.empty_photons(n, device=dev)
+        ctx.init_disk(ph, 0.1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = ctx.trace(ph, args.max_steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        esc = (ph["phase"] == grt_amd.PHASE_REJECTED) & (ph["terminate"] != 0)
+        out[name] = {"packets": n, "wall_s": wall,
+                     "steps_per_s": st["accepted_steps"] / wall,
+                     "packets_per_s": n / wall,
+                     "accepted_steps": st["accepted_steps"],
+                     "rejected_attempts": st["rejected_attempts"],
+                     "terminated_fraction": st["terminated"] / max(1, n),
+                     "escaped_or_captured": int(esc.sum().item())}
+        log(f"scattering MC {name}: {n} packets in {wall:.2f} s, "
+            f"{st['accepted_steps'] / wall:.3e} steps/s")
+        del ph, ctx
+        torch.cuda.empty_cache()
+    out["workload"] = ("disk mode, torus a=0.5 on 256x128x256, albedo 1, absorption+emission, "
+                       "Philox seed 1234, RK45 rtol 1e-7")
+    out["mixed_over_fp64"] = out["mixed"]["steps_per_s"] / out["fp64"]["steps_per_s"]
+    return out
+
+
 def cpu_baseline(fields, kw, su, sv, sample, threads, max_steps):
     """The oracle (FP64 C/OpenMP restatement, reference O(n) cell search) on
     the same rays: all of them when sample is 0, else an evenly strided
@@ -274,6 +325,8 @@ def main():
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the FP64-vs-mixed comparison on the 2048^2 a=0.94 image")
     ap.add_argument("--precision-image", type=int, default=2048)
+    ap.add_argument("--scatter-packets", type=float, default=1e8,
+                    help="packets of the scattering Monte Carlo leg (0: skip it)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint/forward cost measurement")
     ap.add_argument("--adjoint-photons", type=int, default=262144)
@@ -415,6 +468,9 @@ def main():
 
     if not args.no_adjoint:
         result["adjoint"] = adjoint_leg(args, log)
+
+    if args.scatter_packets > 0:
+        result["scattering_mc"] = scattering_leg(args, log)
 
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

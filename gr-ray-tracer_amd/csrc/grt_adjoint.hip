@@ -117,7 +117,7 @@ __global__ __launch_bounds__(256, GRT_TRACE_WAVES_PER_EU) void adjoint_fp64(Adjo
 {
     extern __shared__ double lds[];
     GridView G = a.grid;
-    if (kSep && a.use_lds) {
+    if (kSep) {
         for (int t = threadIdx.x; t < a.lds_doubles; t += blockDim.x) lds[t] = G.tab[t];
         __syncthreads();
         G.tab = lds;

@@ -500,7 +500,6 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
     L.n_out = c->d_counters + 1;
     L.stats = c->d_stats;
     const int lds_doubles = (c->has_grid && c->separable) ? (int)c->table_doubles : 0;
-    L.use_lds = lds_doubles > 0 ? 1 : 0;
     L.lds_doubles = lds_doubles;
     L.mixed = c->params.precision == GRT_PRECISION_MIXED ? 1 : 0;
     if (tape_slots > 0) {
@@ -698,7 +697,6 @@ int grt_adjoint_device(grt_ctx* c, const double* dL_dI, uint64_t n, double* grad
     A.density_scale = c->params.density_g_cm3_per_code;
     A.temperature_scale = c->params.temperature_k_per_code;
     const int lds_doubles = c->separable ? (int)c->table_doubles : 0;
-    A.use_lds = lds_doubles > 0 ? 1 : 0;
     A.lds_doubles = lds_doubles;
     GRT_HIP(c, grt::launch_adjoint(A, blocks, block, (size_t)lds_doubles * sizeof(double), stream));
     GRT_HIP(c, hipStreamSynchronize(stream));

@@ -200,6 +200,63 @@ GRT_HD void ks_rhs(const double s[8], double a, double m, double out[8])
     out[7] = dH_z * lp2 + hl * dlp_z;
 }
 
+// Hamilton's equations of ks_rhs in fewer operations (mixed-precision form).
+// Only the contractions the equations need are formed:
+//   d_k(l.p) = (the explicit x^k dependence) + dr/dx^k * C,
+//     C = U/B - 2 r (r U + a V)/B^2 - W/r^2,  U = x p1 + y p2,
+//     V = y p1 - x p2, W = z p3, B = r^2 + a^2,
+//   with explicit parts (r p1 - a p2)/B, (r p2 + a p1)/B, p3/r;
+//   d_k H = H (3 dr_k / r - dD_k / D), D = r^4 + a^2 z^2,
+//     dD_k = 4 r^3 dr_k + 2 a^2 z delta_kz,
+// instead of the nine d_k l_j and the quotient form of d_k H
+// (kerr_schild_core.hpp:59-95, 161-192): about 100 operations, four
+// reciprocals and two square roots against 248 operations, 21 divisions
+// and two square roots.  The result equals ks_rhs up to rounding.
+GRT_HD void ks_rhs_fast(const double s[8], double a, double m, double out[8])
+{
+    const double x = s[1], y = s[2], z = s[3];
+    const double p0 = s[4], p1 = s[5], p2 = s[6], p3 = s[7];
+    const double a2 = a * a;
+    const double A = (x * x + y * y + z * z) - a2;
+    const double az = a * z;
+    const double S = sqrt(A * A + 4.0 * (az * az));
+    const double r = sqrt(0.5 * (A + S));
+    const double iS = 1.0 / S, ir = 1.0 / r;
+    const double r2 = r * r;
+    const double iB = 1.0 / (r2 + a2);
+    const double r3 = r2 * r;
+    const double iD = 1.0 / (r3 * r + az * az);
+    const double f = (1.0 + A * iS) * (0.5 * ir);
+    const double drx = x * f, dry = y * f;
+    const double drz = z * f + a2 * z * iS * ir;
+    const double U = x * p1 + y * p2, V = y * p1 - x * p2, W = z * p3;
+    const double Q = (r * U + a * V) * iB;          // l1 p1 + l2 p2
+    const double lp = (-p0 + Q) + W * ir;
+    const double H = m * r3 * iD;
+    const double twoH = 2.0 * H;
+    const double hl = twoH * lp;
+    out[0] = -p0 + hl;
+    out[1] = p1 - hl * (r * x + a * y) * iB;
+    out[2] = p2 - hl * (r * y - a * x) * iB;
+    out[3] = p3 - hl * z * ir;
+    const double Cb = U * iB - 2.0 * r * Q * iB - W * ir * ir;
+    const double dlp_x = (r * p1 - a * p2) * iB + drx * Cb;
+    const double dlp_y = (r * p2 + a * p1) * iB + dry * Cb;
+    const double dlp_z = p3 * ir + drz * Cb;
+    const double g = 3.0 * ir - 4.0 * r3 * iD;       // dH_k / H = g dr_k - (2 a^2 z / D) delta_kz
+    const double lp2 = lp * lp;
+    out[5] = H * (g * drx) * lp2 + hl * dlp_x;
+    out[6] = H * (g * dry) * lp2 + hl * dlp_y;
+    out[7] = H * (g * drz - 2.0 * a2 * z * iD) * lp2 + hl * dlp_z;
+}
+
+struct RhsFast {
+    GRT_HD static void eval(const double s[8], const StepParams& P, double out[8])
+    {
+        ks_rhs_fast(s, P.a, P.m, out);
+    }
+};
+
 // Right-hand-side policy of the integrators: the FP64 evaluation above.
 struct RhsExact {
     GRT_HD static void eval(const double s[8], const StepParams& P, double out[8])

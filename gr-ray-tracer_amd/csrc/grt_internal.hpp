@@ -57,8 +57,7 @@ struct TraceLaunch {
     unsigned long long* stats;  // kStatCount slots
     int budget;               // functor calls per photon in this launch
     int step0;                // global step index of the first call of this launch
-    int use_lds;              // stage the separable search tables in LDS
-    int lds_doubles;          // size of that staging area
+    int lds_doubles;          // separable search tables staged in LDS (doubles)
     int mixed;                // GRT_PRECISION_MIXED: run the mixed FP32/FP64 kernel
     // differentiable form: geodesic checkpoints (tape != nullptr selects the
     // checkpointing kernel)
@@ -85,7 +84,6 @@ struct AdjointLaunch {
     double* grad_T;
     double* grad_coeff;       // [absorption, emission]
     double density_scale, temperature_scale;
-    int use_lds;
     int lds_doubles;
 };
 

@@ -7,12 +7,13 @@
 // would move the accept/reject decisions and step sizes by ~1e-3 relative at
 // rtol = 1e-7.  The image-mode emission point and the camera arrival point
 // are RK step endpoints, so such a step sequence changes most pixels.
-// FP64 is therefore kept for the geodesic, with relaxed rounding: this file
-// is compiled with FMA contraction, reciprocal-based division
-// (-freciprocal-math, and the RHS's quotients as products with four
-// reciprocals) and the hardware's approximate sqrt / rsq / rcp sequences
-// (-fapprox-func).  These change the result only at the ulp level, which
-// moves step sizes by ~1e-8 relative.
+// FP64 is therefore kept for the geodesic, with relaxed rounding: the
+// right-hand side is ks_rhs_fast (the same equations in about 100
+// operations instead of 248, with four reciprocals instead of 21 divisions),
+// and this file is compiled with FMA contraction, reciprocal-based division
+// (-freciprocal-math) and the hardware's approximate sqrt / rsq / rcp
+// sequences (-fapprox-func).  These change the result only at the ulp
+// level, which moves step sizes by ~1e-8 relative.
 //
 // FP32 is used where nothing feeds back into the step sequence: the fluid
 // tetrad (Gram-Schmidt under the Kerr-Schild metric), the attenuation factor
@@ -35,7 +36,7 @@ namespace {
 template <int kIntegrator, int kSep>
 __global__ __launch_bounds__(256, GRT_TRACE_WAVES_PER_EU) void trace_mixed(TraceLaunch a)
 {
-    trace_body<kIntegrator, kSep, RhsExact, 1>(a);
+    trace_body<kIntegrator, kSep, RhsFast, 1>(a);
 }
 
 }  // namespace

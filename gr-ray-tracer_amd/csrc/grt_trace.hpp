@@ -167,8 +167,10 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a)
 {
     extern __shared__ double lds[];
     GridView G = a.grid;
-    if (kSep && a.use_lds) {
-        // Stage the separable search tables in LDS.
+    if (kSep) {
+        // Stage the separable search tables in LDS (they always fit: larger
+        // tables make the grid take the general path).  An unconditional
+        // G.tab = lds lets the compiler address the searches as LDS.
         for (int t = threadIdx.x; t < a.lds_doubles; t += blockDim.x) lds[t] = G.tab[t];
         __syncthreads();
         G.tab = lds;

@@ -132,4 +132,14 @@ int harness_trace(const grt_params* p, int nr, int nth, int nph, const double* r
     return 0;
 }
 
+// Both right-hand-side evaluations of the device code on n states (8 each).
+void harness_rhs(const double* states, int n, double a, double m, double* exact, double* fast)
+{
+    for (int q = 0; q < n; ++q) {
+        grt::ks_rhs(states + 8 * q, a, m, exact + 8 * q);
+        grt::ks_rhs_fast(states + 8 * q, a, m, fast + 8 * q);
+        exact[8 * q + 4] = fast[8 * q + 4] = 0.0;
+    }
+}
+
 }  // extern "C"

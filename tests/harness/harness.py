@@ -38,6 +38,18 @@ def lib():
     return _lib
 
 
+def rhs(states, a, m):
+    """(ks_rhs, ks_rhs_fast) of the device code, compiled for the CPU."""
+    L = lib()
+    st = np.ascontiguousarray(states, dtype=np.float64).reshape(-1, 8)
+    ex = np.zeros_like(st)
+    fa = np.zeros_like(st)
+    L.harness_rhs.restype = None
+    L.harness_rhs.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_void_p, C.c_void_p]
+    L.harness_rhs(st.ctypes.data, len(st), a, m, ex.ctypes.data, fa.ctypes.data)
+    return ex, fa
+
+
 def _grid_args(fields):
     if fields is None:
         return [0, 0, 0] + [None] * 7, []

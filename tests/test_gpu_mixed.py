@@ -111,3 +111,81 @@ def test_mixed_rk4_and_general_grid(gpu):
     e = pixel_errors(im["image"][0], i64["image"][0])
     print(e)
     assert e["median_rel_err"] <= MEDIAN_TOL
+
+
+def chi2_same(a, b, edges):
+    """Two-sample chi-squared p-value of two histograms on the same bins."""
+    from scipy import stats
+    ha, _ = np.histogram(a, edges)
+    hb, _ = np.histogram(b, edges)
+    # scale for unequal sample sizes (Press et al., Numerical Recipes 14.3.2)
+    ka, kb = np.sqrt(hb.sum() / ha.sum()), np.sqrt(ha.sum() / hb.sum())
+    m = (ha + hb) > 0
+    chi2 = (((ka * ha - kb * hb)[m]) ** 2 / (ha + hb)[m]).sum()
+    return stats.chi2.sf(chi2, m.sum() - 1)
+
+
+def test_mixed_scattering_statistics(oracle, gpu):
+    """Batched scattering Monte Carlo in disk mode at the settings of the
+    scattering configuration (a = 0.5 torus, albedo 1, absorption and
+    emission on, RK45 rtol 1e-7) on a 32 x 16 x 32 grid, density scale
+    0.125 g/cm^3 (about half the packets scatter).  (1) On the same packets
+    and the same Philox stream (seed 1234) the mixed kernel follows the FP64
+    oracle's paths: most packets leave in the same direction.  (2) Against
+    an independent oracle sample (seed 4321) the escape-direction and
+    log Stokes-I histograms agree by chi-squared, p > 0.01."""
+    import grt_amd
+    from grt_amd.fm_torus import fishbone_moncrief
+    f = fishbone_moncrief(nr=32, ntheta=16, nphi=32, r_min=2.0, r_max=100.0, mass=1.0,
+                          spin=0.5, r_in=6.0, r_pressure_max=12.0)
+    units = {"density_g_cm3_per_code": 0.125}
+
+    def kwargs(seed):
+        return c1_params(spin=0.5, mode=1, atol=1e-7, rtol=1e-7,
+                         superphoton_count_normalization=10.0, scattering_enabled=1, albedo=1.0,
+                         seed=seed, r_term_max=1000.0, absorption_coefficient=1.0,
+                         emission_coefficient=1.0)
+
+    def oracle_run(seed):
+        po = oracle.make_params(**kwargs(seed))
+        G = oracle.GridHolder(f, 0.5, 1.0, units)
+        o0 = oracle.init_disk(po, G, 0.1)
+        o = {k: v.copy() for k, v in o0.items()}
+        oracle.trace(po, G, o, 20000, 1.0, 0, 8)
+        return o0, o
+
+    o0, o = oracle_run(1234)
+    _, o2 = oracle_run(4321)
+    n = len(o0["x0"])
+    assert n > 5000
+    ctx = grt_amd.Context(grt_amd.default_params(precision=grt_amd.PRECISION_MIXED,
+                                                 **kwargs(1234), **units))
+    ctx.set_grid(f)
+    g = grt_amd.to_device(o0)
+    st = ctx.trace(g, 20000)
+    g = grt_amd.to_host(g)
+    esc_g, esc_o, esc_2 = g["phase"] == 4, o["phase"] == 4, o2["phase"] == 4
+    print("packets", n, "escaped", esc_g.sum(), esc_o.sum(), esc_2.sum())
+
+    def direction(P, m):
+        r = np.sqrt(P["x1"] ** 2 + P["x2"] ** 2 + P["x3"] ** 2)[m]
+        return P["x3"][m] / r, np.arctan2(P["x2"][m], P["x1"][m])
+
+    both = esc_g & esc_o
+    cg, pg = direction(g, both)
+    co, pc = direction(o, both)
+    same = (np.abs(cg - co) < 1e-3) & (np.abs(np.angle(np.exp(1j * (pg - pc)))) < 1e-3)
+    print("same-stream packets leaving in the same direction:", same.mean())
+    assert both.mean() > 0.9 and same.mean() > 0.9
+
+    cg, pg = direction(g, esc_g)
+    c2, p2 = direction(o2, esc_2)
+    ps = [chi2_same(cg, c2, np.linspace(-1, 1, 11)),
+          chi2_same(pg, p2, np.linspace(-np.pi, np.pi, 9))]
+    lit_g = esc_g & (g["I"] > 0)
+    lit_2 = esc_2 & (o2["I"] > 0)
+    lo, hi = np.percentile(np.log10(o2["I"][lit_2]), [1, 99])
+    ps.append(chi2_same(np.log10(g["I"][lit_g]), np.log10(o2["I"][lit_2]),
+                        np.linspace(lo, hi, 11)))
+    print("chi2 p-values against an independent sample", ps)
+    assert min(ps) > 0.01
