@@ -110,6 +110,46 @@ GRT_HD double div_rn(double n, double d, double y)
     return fma(res, y, q);
 }
 
+// Correctly rounded sqrt(x) and 1/d for finite, positive, normal-range
+// arguments (2^-700 < x, d < 2^700).  On the device these are the cores of
+// the sequences the compiler emits for sqrt() and 1.0 / d (a v_rsq_f64 /
+// v_rcp_f64 seed refined by Newton steps and a final exact-residual
+// correction) without the exponent pre-scaling and special-value fix-up
+// those carry for denormal, huge, zero or non-finite arguments: for the
+// arguments here the full sequences reduce to exactly these operations, so
+// the results are the same correctly rounded values, bit for bit.
+GRT_HD double sqrt_rn(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+#else
+    return sqrt(x);
+#endif
+}
+
+GRT_HD double rcp_rn(double d)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-d, y, 1.0);
+    y = fma(y, e, y);
+    const double r = fma(-d, y, 1.0);
+    return fma(r, y, y);
+#else
+    return 1.0 / d;
+#endif
+}
+
 // Hamilton's equations for H = 1/2 g^{mu nu} p_mu p_nu in Cartesian
 // Kerr-Schild form, kerr_schild_core.hpp:161-192 (with r, dr/dx^k of :36-46,
 // l, dl/dx^k of :59-77 and H, dH/dx^k of :86-95 expanded for k = x, y, z).
@@ -126,8 +166,10 @@ GRT_HD void ks_rhs(const double s[8], double a, double m, double out[8])
 
     const double A = (x * x + y * y + z * z) - a2;
     const double az = a * z;
-    const double S = sqrt(A * A + 4.0 * (az * az));
-    const double r = sqrt(0.5 * (A + S));
+    // S > 0 and r > 0 away from the ring singularity, which photons never
+    // reach (they terminate at r_term_min)
+    const double S = sqrt_rn(A * A + 4.0 * (az * az));
+    const double r = sqrt_rn(0.5 * (A + S));
     const double tr = 2.0 * r;
     const double B = r * r + a2;
     const double r2 = r * r;
@@ -151,7 +193,7 @@ GRT_HD void ks_rhs(const double s[8], double a, double m, double out[8])
     // Markstein's correction (div_rn), so the five divisors cost one
     // division each instead of one per quotient.  1/(2r) = (1/r)/2 exactly.
     const double D2 = D * D;
-    const double iS = 1.0 / S, ir = 1.0 / r, iB = 1.0 / B, iD = 1.0 / D, iD2 = 1.0 / D2;
+    const double iS = rcp_rn(S), ir = rcp_rn(r), iB = rcp_rn(B), iD = rcp_rn(D), iD2 = rcp_rn(D2);
     const double itr = 0.5 * ir;
     const double f = 1.0 + div_rn(A, S, iS);
     const double drx = div_rn(x * f, tr, itr);

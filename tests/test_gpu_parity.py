@@ -272,3 +272,24 @@ def test_autotune_keeps_results(oracle, gpu, c1):
             assert np.array_equal(h[k], ref[k]), (tune, k)
         if tune["sub_budget"] == 8:
             assert st["launches"] > 1
+
+
+def test_correctly_rounded_sqrt_and_reciprocal(gpu):
+    """sqrt_rn / rcp_rn of the FP64 kernel equal the compiler's sqrt() and
+    1.0 / d bit for bit: 2^27 pseudo-random arguments over binary exponents
+    [-700, 700] and edge values (powers of two, neighbours of 1, the ends
+    of the range, values the right-hand side meets)."""
+    import ctypes as C
+    import os
+    here = os.path.join(os.path.dirname(__file__), "harness", "libgrt_math_check.so")
+    L = C.CDLL(here)
+    L.grt_math_check.restype = C.c_int
+    L.grt_math_check.argtypes = [C.c_uint64, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p]
+    edge = [2.0 ** k for k in range(-700, 701, 7)]
+    edge += [np.nextafter(1.0, 0), 1.0, np.nextafter(1.0, 2), 2.0 ** -700, 2.0 ** 700 * 1.9999,
+             0.81, 1.0 + 2 ** -52, 4.0 - 2 ** -50, 1e-30, 1e30, 123456.789, 2.5e25]
+    v = np.ascontiguousarray(edge, dtype=np.float64)
+    bad = (C.c_ulonglong * 4)()
+    assert L.grt_math_check(1 << 27, 12345, v.ctypes.data, len(v), bad) == 0
+    print("mismatches sqrt/rcp random", bad[0], bad[1], "edge", bad[2], bad[3])
+    assert list(bad) == [0, 0, 0, 0]
