@@ -154,9 +154,6 @@ GRT_HD double rcp_rn(double d)
 // Kerr-Schild form, kerr_schild_core.hpp:161-192 (with r, dr/dx^k of :36-46,
 // l, dl/dx^k of :59-77 and H, dH/dx^k of :86-95 expanded for k = x, y, z).
 // out[4] (dp_t/dlambda) is identically zero and is not written.
-#ifndef GRT_RHS_RECIPROCALS
-#define GRT_RHS_RECIPROCALS 0
-#endif
 
 GRT_HD void ks_rhs(const double s[8], double a, double m, double out[8])
 {
@@ -175,19 +172,6 @@ GRT_HD void ks_rhs(const double s[8], double a, double m, double out[8])
     const double r2 = r * r;
     const double r3 = r2 * r;
     const double D = r3 * r + az * az;
-#if GRT_RHS_RECIPROCALS
-    // (mixed-precision form) Four reciprocals replace the 21 divisions and
-    // each quotient is a plain product with one of them, within an ulp or
-    // two of the reference's quotient.
-    const double iS = 1.0 / S, itr = 1.0 / tr, iB = 1.0 / B, iD = 1.0 / D;
-    const double ir = 2.0 * itr;
-    const double iD2 = iD * iD;
-    const double f = 1.0 + A * iS;
-    const double drx = x * f * itr;
-    const double dry = y * f * itr;
-    const double drz = (z * f + 2.0 * a2 * z * iS) * itr;
-#define GRT_DIV(n, d, inv) ((n) * (inv))
-#else
     // Every quotient n/d is the correctly rounded IEEE quotient, evaluated
     // from the correctly rounded reciprocal y = 1/d of its divisor with
     // Markstein's correction (div_rn), so the five divisors cost one
@@ -199,31 +183,28 @@ GRT_HD void ks_rhs(const double s[8], double a, double m, double out[8])
     const double drx = div_rn(x * f, tr, itr);
     const double dry = div_rn(y * f, tr, itr);
     const double drz = div_rn(z * f + div_rn(2.0 * a2 * z, S, iS), tr, itr);
-#define GRT_DIV(n, d, inv) div_rn((n), (d), (inv))
-#endif
 
-    const double l1 = GRT_DIV(r * x + a * y, B, iB);
-    const double l2 = GRT_DIV(r * y - a * x, B, iB);
-    const double l3 = GRT_DIV(z, r, ir);
+    const double l1 = div_rn(r * x + a * y, B, iB);
+    const double l2 = div_rn(r * y - a * x, B, iB);
+    const double l3 = div_rn(z, r, ir);
     const double dBx = tr * drx, dBy = tr * dry, dBz = tr * drz;
     // dl[k][mu]; the Kronecker terms of :73-75 reduce to +-r, +-a or 1.
-    const double dl_x1 = GRT_DIV((drx * x + r) - l1 * dBx, B, iB);
-    const double dl_x2 = GRT_DIV((drx * y - a) - l2 * dBx, B, iB);
-    const double dl_x3 = GRT_DIV(-(l3 * drx), r, ir);
-    const double dl_y1 = GRT_DIV((dry * x + a) - l1 * dBy, B, iB);
-    const double dl_y2 = GRT_DIV((dry * y + r) - l2 * dBy, B, iB);
-    const double dl_y3 = GRT_DIV(-(l3 * dry), r, ir);
-    const double dl_z1 = GRT_DIV(drz * x - l1 * dBz, B, iB);
-    const double dl_z2 = GRT_DIV(drz * y - l2 * dBz, B, iB);
-    const double dl_z3 = GRT_DIV(1.0 - l3 * drz, r, ir);
+    const double dl_x1 = div_rn((drx * x + r) - l1 * dBx, B, iB);
+    const double dl_x2 = div_rn((drx * y - a) - l2 * dBx, B, iB);
+    const double dl_x3 = div_rn(-(l3 * drx), r, ir);
+    const double dl_y1 = div_rn((dry * x + a) - l1 * dBy, B, iB);
+    const double dl_y2 = div_rn((dry * y + r) - l2 * dBy, B, iB);
+    const double dl_y3 = div_rn(-(l3 * dry), r, ir);
+    const double dl_z1 = div_rn(drz * x - l1 * dBz, B, iB);
+    const double dl_z2 = div_rn(drz * y - l2 * dBz, B, iB);
+    const double dl_z3 = div_rn(1.0 - l3 * drz, r, ir);
 
-    const double H = GRT_DIV(m * r3, D, iD);
+    const double H = div_rn(m * r3, D, iD);
     const double c3 = 3.0 * r2;
     const double c4 = 4.0 * r3;
-    const double dH_x = GRT_DIV(m * (c3 * drx * D - r3 * (c4 * drx)), D2, iD2);
-    const double dH_y = GRT_DIV(m * (c3 * dry * D - r3 * (c4 * dry)), D2, iD2);
-    const double dH_z = GRT_DIV(m * (c3 * drz * D - r3 * (c4 * drz + 2.0 * a2 * z)), D2, iD2);
-#undef GRT_DIV
+    const double dH_x = div_rn(m * (c3 * drx * D - r3 * (c4 * drx)), D2, iD2);
+    const double dH_y = div_rn(m * (c3 * dry * D - r3 * (c4 * dry)), D2, iD2);
+    const double dH_z = div_rn(m * (c3 * drz * D - r3 * (c4 * drz + 2.0 * a2 * z)), D2, iD2);
 
     // l^mu = (-1, l1, l2, l3)
     const double lp = ((-p0 + l1 * p1) + l2 * p2) + l3 * p3;

@@ -24,8 +24,7 @@ HEADERS = ["grt_device.hpp", "grt_internal.hpp", "grt_trace.hpp"]
 # operation for operation: it may contract a * b + c into one FMA, divide by
 # multiplying with a reciprocal and use the hardware's approximate
 # sqrt / exp / reciprocal instructions.
-RELAXED_FLAGS = {"grt_mixed.hip": ["-freciprocal-math", "-fapprox-func",
-                                   "-DGRT_RHS_RECIPROCALS=1"]}
+RELAXED_FLAGS = {"grt_mixed.hip": ["-freciprocal-math", "-fapprox-func"]}
 
 
 def hipcc():
