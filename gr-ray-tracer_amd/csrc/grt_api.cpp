@@ -27,6 +27,7 @@ struct grt_ctx {
     grt_tuning tune{256, 2, 256};
     int device = 0;
     int num_cus = 0;
+    size_t max_lds = 65536;   // dynamic LDS one workgroup may use
     std::string err;
 
     // grid
@@ -334,6 +335,7 @@ int grt_create(const grt_params* p, grt_ctx** out)
     grt_ctx* c = new grt_ctx();
     c->device = dev;
     c->num_cus = prop.multiProcessorCount;
+    c->max_lds = prop.sharedMemPerBlock;
     c->params = *p;
     c->sp = grt::make_step_params(*p);
     if (hipMalloc(&c->d_counters, 8 * sizeof(uint32_t)) != hipSuccess ||
@@ -462,7 +464,7 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
 {
     if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
     if (!dev || !photons_complete(dev)) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
-    if (n > 0xFFFFFFF0ull) return fail(c, GRT_E_INVALID, "more than 2^32 - 16 photons in one call");
+    if (n > 0x80000000ull) return fail(c, GRT_E_INVALID, "more than 2^31 photons in one call");
     if (max_steps <= 0) return fail(c, GRT_E_INVALID, "integration.max_steps must be positive");
     if (!(termination_percent > 0.0) || termination_percent > 1.0)
         return fail(c, GRT_E_INVALID, "integration.termination_percent must be in (0, 1]");
@@ -526,8 +528,14 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
         L.mixed = 0;
         c->tape_n = 0;
     }
-    const size_t lds_bytes = (size_t)lds_doubles * sizeof(double);
     const int block = c->tune.block;
+    const size_t lds_bytes = grt::trace_lds_bytes(lds_doubles, block);
+    if (lds_bytes > c->max_lds)
+        return fail(c, GRT_E_INVALID, "the grid's search tables and the photon stages of a " +
+                                          std::to_string(block) + "-thread workgroup need " +
+                                          std::to_string(lds_bytes) + " B of LDS, more than the " +
+                                          std::to_string(c->max_lds) +
+                                          " B a workgroup may use; lower tuning.block");
     const int persistent_blocks = std::max(1, c->num_cus * c->tune.blocks_per_cu);
 
     // Global-step schedule, integrate_cart_ks.hpp:468-527.  When the stop
@@ -553,7 +561,9 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
         int64_t step0 = g;
         while (left > 0 && live > 0) {
             const int budget = (int)std::min<int64_t>(left, c->tune.sub_budget);
-            L.list_in = list[cur];
+            // With every photon live the list is a permutation of all
+            // photons, and the order of processing does not change results.
+            L.list_in = (launches == 0 && live == n) ? nullptr : list[cur];
             L.list_out = list[cur ^ 1];
             L.n_in = live;
             L.budget = budget;

@@ -45,11 +45,35 @@ struct TraceConsts {
     StepParams P;
 };
 
+// One wavefront's photon stage in LDS (grt_trace.hpp): the photons of the
+// next (up to 64) work-list slots, loaded with one coalesced pass over the
+// photon arrays, so a lane that needs a new photon reads it from LDS instead of
+// waiting for HBM.  Placed after the search tables, one per wavefront.
+constexpr int kStageDoubles = 15;   // s[8], stokes[4], freq, eem, dl
+struct WaveStage {
+    double v[kStageDoubles][64];
+    uint64_t id[64];
+    uint32_t draw[64];
+    uint32_t idx[64];
+    uint32_t phase[64];
+    uint32_t base;      // work-list slot of stage position 0
+    uint32_t next;      // next slot to hand out, base <= next <= end
+    uint32_t end;       // one past the last staged slot, end <= base + 64
+    uint32_t drained;   // the work counter has passed the end of the list
+};
+
+// Dynamic LDS of a tracing launch: search tables, then one stage per wavefront.
+inline size_t trace_lds_bytes(int lds_doubles, int block)
+{
+    const size_t tables = (((size_t)lds_doubles + 1) & ~(size_t)1) * sizeof(double);
+    return tables + (size_t)(block / 64) * sizeof(WaveStage);
+}
+
 struct TraceLaunch {
     const TraceConsts* consts;
     GridView grid;
     int integrator;
-    const uint32_t* list_in;
+    const uint32_t* list_in;  // nullptr: the identity list (slot i is photon i)
     uint32_t n_in;
     uint32_t* queue;          // work counter over list_in, zeroed before launch
     uint32_t* list_out;       // photons still live after `budget` calls

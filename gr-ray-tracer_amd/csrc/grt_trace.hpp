@@ -5,11 +5,13 @@
 // One photon per lane.  Each lane keeps its photon entirely in registers and
 // loops over RK45 attempts; when its photon terminates or uses up this
 // launch's step budget the lane writes the photon back and takes the next
-// index from the launch's work list with one atomic per wavefront (ballot +
-// mbcnt), so short paths never idle a lane while long paths in the same
-// wavefront continue.  Photons still live after the budget are appended to
-// the next launch's list, which bounds the tail of every launch (compaction
-// between launches).
+// photon of the launch's work list from its wavefront's stage in LDS (the
+// next slots of the list, claimed with one atomic and loaded with one
+// coalesced pass), so short paths never idle a lane while long paths in the
+// same wavefront continue, and a refill costs an LDS read instead of an HBM
+// round trip.  Photons still live after the budget are appended to the next
+// launch's list, which bounds the tail of every launch (compaction between
+// launches).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -45,22 +47,96 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
-__device__ __forceinline__ void load_photon(const PhotonPtrs& p, uint32_t idx, Photon& ph,
-                                            PhotonRng& rng)
+// Fills stage positions [0, count) with work-list slots base + position,
+// lane l loading position l; slots at or past n_in stay empty.  Called with
+// all 64 lanes of the wavefront active, so each photon array is read in one
+// coalesced pass.
+__device__ __forceinline__ void stage_fill(WaveStage* S, const TraceLaunch& a,
+                                           const PhotonPtrs& p, uint32_t base, uint32_t count)
 {
+    const uint32_t l = threadIdx.x & 63;
+    const uint32_t slot = base + l;
+    if (l >= count || slot >= a.n_in) return;
+    const uint32_t i = a.list_in ? a.list_in[slot] : slot;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        ph.s[q] = p.x[q][idx];
-        ph.s[4 + q] = p.k[q][idx];
-        ph.stokes[q] = p.st[q][idx];
+        S->v[q][l] = p.x[q][i];
+        S->v[4 + q][l] = p.k[q][i];
+        S->v[8 + q][l] = p.st[q][i];
     }
-    ph.freq = p.freq[idx];
-    ph.eem = p.eem[idx];
-    ph.dl = p.dl[idx];
-    ph.phase = p.phase[idx];
+    S->v[12][l] = p.freq[i];
+    S->v[13][l] = p.eem[i];
+    S->v[14][l] = p.dl[i];
+    S->id[l] = p.id[i];
+    S->draw[l] = p.rng_draw[i];
+    S->phase[l] = p.phase[i];
+    S->idx[l] = i;
+}
+
+// Reads the photon at stage position `pos` into the lane; returns its index.
+__device__ __forceinline__ uint32_t stage_read(const WaveStage* S, uint32_t pos, Photon& ph,
+                                               PhotonRng& rng)
+{
+#pragma unroll
+    for (int q = 0; q < 8; ++q) ph.s[q] = S->v[q][pos];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) ph.stokes[q] = S->v[8 + q][pos];
+    ph.freq = S->v[12][pos];
+    ph.eem = S->v[13][pos];
+    ph.dl = S->v[14][pos];
+    ph.phase = (uint8_t)S->phase[pos];
     ph.term = 0;
-    rng.id = p.id[idx];
-    rng.draw = p.rng_draw[idx];
+    rng.id = S->id[pos];
+    rng.draw = S->draw[pos];
+    return S->idx[pos];
+}
+
+// Hands the next work-list slots, in order, to the lanes with `need` set.
+// When the stage runs out it claims the next slots with one atomic on the
+// work counter and loads them: 64 at a time while more than two photons per
+// lane of the grid remain, then only as many as the wavefront needs, so the
+// end of the list is spread over the wavefronts as evenly as single claims
+// would.  Must be called with all 64 lanes active.  Returns whether this
+// lane received a photon (then ph, rng and idx hold it).
+__device__ __forceinline__ bool stage_take(WaveStage* S, const TraceLaunch& a,
+                                           const PhotonPtrs& p, bool need, Photon& ph,
+                                           PhotonRng& rng, uint32_t& idx)
+{
+    volatile WaveStage* V = S;
+    const uint64_t mask = __ballot(need);
+    const uint32_t total = (uint32_t)__popcll(mask);
+    const uint32_t rank = lane_rank(mask);
+    const uint32_t base = V->base, next = V->next, end = V->end;
+    const uint32_t avail = end - next;
+    bool got = false;
+    if (need && rank < avail && next + rank < a.n_in) {
+        idx = stage_read(S, next + rank - base, ph, rng);
+        got = true;
+    }
+    uint32_t nbase = base, nnext = next + (total < avail ? total : avail), nend = end;
+    if (total > avail && !V->drained) {
+        const uint32_t lanes = gridDim.x * blockDim.x;
+        const uint32_t want =
+            ((uint64_t)end + 2ull * lanes < (uint64_t)a.n_in) ? 64u : total - avail;
+        uint32_t c = 0;
+        if ((threadIdx.x & 63) == 0) c = atomicAdd(a.queue, want);
+        nbase = __builtin_amdgcn_readfirstlane(c);
+        stage_fill(S, a, p, nbase, want);
+        const uint32_t slot = nbase + (rank - avail);
+        if (need && rank >= avail && slot < a.n_in) {
+            idx = stage_read(S, slot - nbase, ph, rng);
+            got = true;
+        }
+        nnext = nbase + (total - avail);
+        nend = nbase + want;
+        if ((threadIdx.x & 63) == 0 && (uint64_t)nend >= (uint64_t)a.n_in) V->drained = 1u;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        V->base = nbase;
+        V->next = nnext;
+        V->end = nend;
+    }
+    return got;
 }
 
 __device__ __forceinline__ void store_photon(const PhotonPtrs& p, uint32_t idx, const Photon& ph,
@@ -178,6 +254,17 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a)
     const StepParams& P = a.consts->P;
     const PhotonPtrs& PH = a.consts->ph;
 
+    // This wavefront's photon stage follows the tables.
+    WaveStage* S = reinterpret_cast<WaveStage*>(lds + ((a.lds_doubles + 1) & ~1)) +
+                   (threadIdx.x >> 6);
+    if ((threadIdx.x & 63) == 0) {
+        volatile WaveStage* V = S;
+        V->base = 0u;
+        V->next = 0u;
+        V->end = 0u;
+        V->drained = 0u;
+    }
+
     LaneCounters n{0u, 0u, 0u, 0u, 0u};
     uint32_t last_term = 0;
 
@@ -185,54 +272,61 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a)
     PhotonRng rng;
     rng.seed = P.seed;
     rng.stream = P.stream << 1;
-    uint32_t slot = wave_claim(a.queue);
-    bool have = slot < a.n_in;
-    uint32_t idx = have ? a.list_in[slot] : 0u;
+    uint32_t idx = 0;
+    bool have = stage_take(S, a, PH, true, L.ph, rng, idx);
     uint64_t thdr = 0;
-    if (have) {
-        load_photon(PH, idx, L.ph, rng);
-        if (kTape) thdr = a.tape_hdr[idx];
-    }
+    if (kTape && have) thdr = a.tape_hdr[idx];
     lane_reset(L);
     int calls_left = a.budget;
     int calls_done = 0;
 
-    while (have) {
-        double au = 0.0, av = 0.0;
-        if (kTape && !L.in_call && ((uint32_t)thdr % kSegmentCalls) == 0) {
-            const uint32_t k = (uint32_t)(thdr >> 32) & 0x7FFFFFFFu;
-            if (k < (uint32_t)a.tape_slots)
-                write_checkpoint(a.tape + ((size_t)idx * a.tape_slots + k) * kCkptDoubles, L.ph,
-                                 rng, (uint32_t)thdr);
-        }
-        TapeFlag flag;
-        bool call_done;
-        if (kTape)
-            call_done = lane_iterate<kIntegrator, kSep, Rhs, kMixed, TapeFlag>(L, G, P, rng, n,
-                                                                              au, av, flag);
-        else
-            call_done = lane_iterate<kIntegrator, kSep, Rhs, kMixed>(L, G, P, rng, n, au, av);
-        if (kTape && flag.touched) thdr |= kTapeActive;
-        if (call_done) {
-            ++n.calls;
-            ++calls_done;
-            --calls_left;
-            if (kTape) thdr = tape_end_call(thdr, false, L.ph.term != 0, a.tape_slots, a.tape_overflow);
-            if (L.ph.term || calls_left == 0) {
-                store_photon(PH, idx, L.ph, rng, au, av);
-                if (kTape) a.tape_hdr[idx] = thdr;
-                if (L.ph.term) {
-                    const uint32_t step = (uint32_t)(a.step0 + calls_done - 1);
-                    last_term = step > last_term ? step : last_term;
-                } else {
-                    const uint32_t o = wave_claim(a.n_out);
-                    a.list_out[o] = idx;
+    // The loop runs until no lane of the wavefront holds a photon; lanes
+    // without one sit out the step and join the refills, which need the
+    // whole wavefront.
+    while (__ballot(have) != 0) {
+        bool need = false;
+        if (have) {
+            double au = 0.0, av = 0.0;
+            if (kTape && !L.in_call && ((uint32_t)thdr % kSegmentCalls) == 0) {
+                const uint32_t k = (uint32_t)(thdr >> 32) & 0x7FFFFFFFu;
+                if (k < (uint32_t)a.tape_slots)
+                    write_checkpoint(a.tape + ((size_t)idx * a.tape_slots + k) * kCkptDoubles,
+                                     L.ph, rng, (uint32_t)thdr);
+            }
+            TapeFlag flag;
+            bool call_done;
+            if (kTape)
+                call_done = lane_iterate<kIntegrator, kSep, Rhs, kMixed, TapeFlag>(
+                    L, G, P, rng, n, au, av, flag);
+            else
+                call_done = lane_iterate<kIntegrator, kSep, Rhs, kMixed>(L, G, P, rng, n, au, av);
+            if (kTape && flag.touched) thdr |= kTapeActive;
+            if (call_done) {
+                ++n.calls;
+                ++calls_done;
+                --calls_left;
+                if (kTape)
+                    thdr = tape_end_call(thdr, false, L.ph.term != 0, a.tape_slots,
+                                         a.tape_overflow);
+                if (L.ph.term || calls_left == 0) {
+                    store_photon(PH, idx, L.ph, rng, au, av);
+                    if (kTape) a.tape_hdr[idx] = thdr;
+                    if (L.ph.term) {
+                        const uint32_t step = (uint32_t)(a.step0 + calls_done - 1);
+                        last_term = step > last_term ? step : last_term;
+                    } else {
+                        const uint32_t o = wave_claim(a.n_out);
+                        a.list_out[o] = idx;
+                    }
+                    need = true;
                 }
-                slot = wave_claim(a.queue);
-                have = slot < a.n_in;
-                if (have) {
-                    idx = a.list_in[slot];
-                    load_photon(PH, idx, L.ph, rng);
+            }
+        }
+        if (__ballot(need) != 0) {
+            const bool got = stage_take(S, a, PH, need, L.ph, rng, idx);
+            if (need) {
+                have = got;
+                if (got) {
                     if (kTape) thdr = a.tape_hdr[idx];
                     lane_reset(L);
                     calls_left = a.budget;

@@ -144,7 +144,7 @@ int grt_set_grid(grt_ctx *ctx, int32_t nr, int32_t nth, int32_t nph,
  * not reuse draws.
  * grt_trace takes host arrays (copied in and out); grt_trace_device takes
  * device arrays and runs on `stream` (a hipStream_t, NULL = default stream).
- * stats may be NULL.
+ * stats may be NULL.  At most 2^31 photons per call (GRT_E_INVALID beyond).
  */
 int grt_trace(grt_ctx *ctx, const grt_photons *host, uint64_t n, int32_t max_steps,
               double termination_percent, int64_t output_interval, grt_stats *stats);
