@@ -37,9 +37,9 @@ enum StatSlot {
     kStatCount
 };
 
-// Launch constants kept in device memory: the kernel reads them through a
-// pointer at their use sites instead of pinning ~190 scalar registers for the
-// whole persistent loop.
+// Launch constants.  The tracing kernels take them by value in their
+// argument block (constant memory: scalar loads, no wait on the kernel's own
+// photon stores); the adjoint reads the device copy the last trace left.
 struct TraceConsts {
     PhotonPtrs ph;
     StepParams P;
@@ -56,6 +56,10 @@ struct WaveStage {
     uint32_t draw[64];
     uint32_t idx[64];
     uint32_t phase[64];
+};
+
+// Bookkeeping of a wavefront's stage, wave-uniform (kept in scalar registers).
+struct StageState {
     uint32_t base;      // work-list slot of stage position 0
     uint32_t next;      // next slot to hand out, base <= next <= end
     uint32_t end;       // one past the last staged slot, end <= base + 64
@@ -70,7 +74,7 @@ inline size_t trace_lds_bytes(int lds_doubles, int block)
 }
 
 struct TraceLaunch {
-    const TraceConsts* consts;
+    TraceConsts c;
     GridView grid;
     int integrator;
     const uint32_t* list_in;  // nullptr: the identity list (slot i is photon i)

@@ -16,10 +16,10 @@
 // level, which moves step sizes by ~1e-8 relative.
 //
 // FP32 is used where nothing feeds back into the step sequence: the fluid
-// tetrad (Gram-Schmidt under the Kerr-Schild metric), the attenuation factor
-// exp(-dtau) of the radiative-transfer update, and the controller's
-// err^(-1/4).  A photon scattered with the FP32 tetrad is put back on the
-// null cone in FP64.
+// tetrad (Gram-Schmidt under the Kerr-Schild metric) and the attenuation
+// factor exp(-dtau) of the radiative-transfer update.  The controller's
+// err^(-1/4) feeds every step size and stays FP64.  A photon scattered with
+// the FP32 tetrad is put back on the null cone in FP64.
 //
 // On this chip FP32 arithmetic issues at the FP64 rate unless it is packed
 // two per lane, which this build does not use; FP32 pays only where it

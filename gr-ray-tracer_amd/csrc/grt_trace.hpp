@@ -96,45 +96,40 @@ __device__ __forceinline__ uint32_t stage_read(const WaveStage* S, uint32_t pos,
 // work counter and loads them: 64 at a time while more than two photons per
 // lane of the grid remain, then only as many as the wavefront needs, so the
 // end of the list is spread over the wavefronts as evenly as single claims
-// would.  Must be called with all 64 lanes active.  Returns whether this
-// lane received a photon (then ph, rng and idx hold it).
-__device__ __forceinline__ bool stage_take(WaveStage* S, const TraceLaunch& a,
+// would.  Must be called with all 64 lanes active (st stays wave-uniform).
+// Returns whether this lane received a photon (then ph, rng and idx hold it).
+__device__ __forceinline__ bool stage_take(WaveStage* S, StageState& st, const TraceLaunch& a,
                                            const PhotonPtrs& p, bool need, Photon& ph,
                                            PhotonRng& rng, uint32_t& idx)
 {
-    volatile WaveStage* V = S;
     const uint64_t mask = __ballot(need);
     const uint32_t total = (uint32_t)__popcll(mask);
     const uint32_t rank = lane_rank(mask);
-    const uint32_t base = V->base, next = V->next, end = V->end;
-    const uint32_t avail = end - next;
+    const uint32_t avail = st.end - st.next;
     bool got = false;
-    if (need && rank < avail && next + rank < a.n_in) {
-        idx = stage_read(S, next + rank - base, ph, rng);
+    if (need && rank < avail && st.next + rank < a.n_in) {
+        idx = stage_read(S, st.next + rank - st.base, ph, rng);
         got = true;
     }
-    uint32_t nbase = base, nnext = next + (total < avail ? total : avail), nend = end;
-    if (total > avail && !V->drained) {
+    if (total > avail && !st.drained) {
         const uint32_t lanes = gridDim.x * blockDim.x;
         const uint32_t want =
-            ((uint64_t)end + 2ull * lanes < (uint64_t)a.n_in) ? 64u : total - avail;
+            ((uint64_t)st.end + 2ull * lanes < (uint64_t)a.n_in) ? 64u : total - avail;
         uint32_t c = 0;
         if ((threadIdx.x & 63) == 0) c = atomicAdd(a.queue, want);
-        nbase = __builtin_amdgcn_readfirstlane(c);
+        const uint32_t nbase = __builtin_amdgcn_readfirstlane(c);
         stage_fill(S, a, p, nbase, want);
         const uint32_t slot = nbase + (rank - avail);
         if (need && rank >= avail && slot < a.n_in) {
             idx = stage_read(S, slot - nbase, ph, rng);
             got = true;
         }
-        nnext = nbase + (total - avail);
-        nend = nbase + want;
-        if ((threadIdx.x & 63) == 0 && (uint64_t)nend >= (uint64_t)a.n_in) V->drained = 1u;
-    }
-    if ((threadIdx.x & 63) == 0) {
-        V->base = nbase;
-        V->next = nnext;
-        V->end = nend;
+        st.base = nbase;
+        st.next = nbase + (total - avail);
+        st.end = nbase + want;
+        st.drained = (uint64_t)st.end >= (uint64_t)a.n_in ? 1u : 0u;
+    } else {
+        st.next += total < avail ? total : avail;
     }
     return got;
 }
@@ -251,19 +246,13 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a)
         __syncthreads();
         G.tab = lds;
     }
-    const StepParams& P = a.consts->P;
-    const PhotonPtrs& PH = a.consts->ph;
+    const StepParams& P = a.c.P;
+    const PhotonPtrs& PH = a.c.ph;
 
     // This wavefront's photon stage follows the tables.
     WaveStage* S = reinterpret_cast<WaveStage*>(lds + ((a.lds_doubles + 1) & ~1)) +
                    (threadIdx.x >> 6);
-    if ((threadIdx.x & 63) == 0) {
-        volatile WaveStage* V = S;
-        V->base = 0u;
-        V->next = 0u;
-        V->end = 0u;
-        V->drained = 0u;
-    }
+    StageState st{0u, 0u, 0u, 0u};
 
     LaneCounters n{0u, 0u, 0u, 0u, 0u};
     uint32_t last_term = 0;
@@ -273,7 +262,7 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a)
     rng.seed = P.seed;
     rng.stream = P.stream << 1;
     uint32_t idx = 0;
-    bool have = stage_take(S, a, PH, true, L.ph, rng, idx);
+    bool have = stage_take(S, st, a, PH, true, L.ph, rng, idx);
     uint64_t thdr = 0;
     if (kTape && have) thdr = a.tape_hdr[idx];
     lane_reset(L);
@@ -323,7 +312,7 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a)
             }
         }
         if (__ballot(need) != 0) {
-            const bool got = stage_take(S, a, PH, need, L.ph, rng, idx);
+            const bool got = stage_take(S, st, a, PH, need, L.ph, rng, idx);
             if (need) {
                 have = got;
                 if (got) {
