@@ -36,6 +36,7 @@ struct grt_ctx {
     bool separable = false;
     double* d_tables = nullptr;   // separable search tables, then the three axes
     size_t table_doubles = 0;
+    int key[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // bucket keys of the separable search
     double* d_coords = nullptr;   // r3, th3, ph3 (general path only)
     double* d_cells = nullptr;    // kCellDoubles per cell
     double* d_rho_code = nullptr; // unscaled density, for disk-mode packet counts
@@ -412,6 +413,7 @@ int grt_set_grid(grt_ctx* c, int32_t nr, int32_t nth, int32_t nph, const double*
     c->nph = nph;
     c->ncell = G;
     c->separable = hg.separable;
+    for (int q = 0; q < 9; ++q) c->key[q] = hg.key[q];
     hipError_t e;
     if (hg.separable) {
         std::vector<double> t = hg.tables;
@@ -449,7 +451,7 @@ GridView make_grid_view(const grt_ctx* c)
 {
     return grt::grid_view(c->nr, c->nth, c->nph, c->has_grid, c->separable, c->d_tables,
                           c->d_tables ? c->d_tables + c->table_doubles : nullptr, c->d_coords,
-                          c->d_cells);
+                          c->d_cells, c->key);
 }
 
 }  // namespace

@@ -461,7 +461,26 @@ struct GridView {
     const double* th3;
     const double* ph3;
     const double* cells;      // kCellDoubles per cell, C order
+    // bucket index of the separable search (axis 0 radial, 1 polar,
+    // 2 azimuth; key_nb 0: none), see axis_bucket
+    int key_shift[3], key_base[3], key_nb[3];
 };
+
+// Bucket of a positive double w for the separable search: the top bits of
+// its IEEE pattern (monotone in w), offset and clamped to [0, nb).  With
+// table entries e_j keyed the same way, every entry in a lower bucket is
+// below w and every entry in a higher bucket above it, so a count of
+// entries <= w needs comparisons only inside w's bucket; the host stores,
+// per bucket b, the number of entries keyed below b (uint16, after the
+// search tables).  NaN goes to bucket 0, where no entry compares true.
+GRT_HD int axis_bucket(double w, int shift, int base, int nb)
+{
+    if (!(w == w)) return 0;
+    uint64_t u;
+    __builtin_memcpy(&u, &w, sizeof u);
+    const int64_t k = (int64_t)(u >> shift) - (int64_t)base;
+    return (int)(k < 0 ? 0 : (k >= nb ? nb - 1 : k));
+}
 
 // scattering.hpp:41-45.  fmod(d, 2 pi) is d itself for 0 <= d < 2 pi, the
 // only range the arguments here reach; the library call is kept for the rest.
@@ -504,6 +523,24 @@ GRT_HD int count_ge_desc(const double* t, int n, double v)
     return lo;
 }
 
+// count_le / count_ge_desc narrowed by the bucket counts B of the table
+// (axis_bucket of the keyed query w); equal to the plain counts.
+GRT_HD int bucket_count_le(const double* t, const uint16_t* B, int shift, int base, int nb,
+                           double w, double v)
+{
+    const int b = axis_bucket(w, shift, base, nb);
+    const int l = B[b], h = B[b + 1];
+    return l + count_le(t + l, h - l, v);
+}
+
+GRT_HD int bucket_count_ge_desc(const double* t, const uint16_t* B, int shift, int base, int nb,
+                                double w, double v)
+{
+    const int b = axis_bucket(w, shift, base, nb);
+    const int l = B[b], h = B[b + 1];
+    return l + count_ge_desc(t + l, h - l, v);
+}
+
 // A monotone stand-in for atan2(Y, X) mapped to [0, 2 pi): the "diamond
 // angle" in [0, 4), one division, no transcendental.
 GRT_HD double pseudo_angle(double X, double Y)
@@ -537,6 +574,8 @@ GRT_HD double pseudo_angle(double X, double Y)
 //   [0,n) r_j   [n,2n) sqrt(r_j r_j+1)   [2n,3n) r_j e^-thr   [3n,4n) r_j e^thr
 //   then cos th_j | cos of midpoints | cos(min(pi, th_j + thr)) | cos(max(0, th_j - thr))
 //   then pseudo(phi_k) | cos phi_k | sin phi_k | cos(thr_k) (-2: single entry)
+//   then the uint16 bucket counts of the three axes (see axis_bucket): keyed
+//   on r, on 3 - cos theta and on pseudo + 4, so that every key is positive.
 GRT_HD bool locate_separable(const GridView& G, double radius, double ct, double X, double Y,
                              int& i_out, int& j_out, int& k_out)
 {
@@ -544,13 +583,18 @@ GRT_HD bool locate_separable(const GridView& G, double radius, double ct, double
     const double* R = G.tab;
     const double* T = R + 4 * nr;
     const double* F = T + 4 * nth;
+    const uint16_t* BR = reinterpret_cast<const uint16_t*>(F + 4 * nph);
+    const uint16_t* BT = BR + G.key_nb[0] + 1;
+    const uint16_t* BF = BT + G.key_nb[1] + 1;
     // radial (scattering.hpp:47-65).  Beyond the outer window of the last
     // cell or below the inner window of the first the nearest cell is that
     // end cell and its window test fails, so no search is needed.
     if (!(radius <= R[4 * nr - 1]) || !(radius >= R[2 * nr])) return false;
     int i;
     {
-        const int lo = count_le(R, nr, radius) - 1;
+        const int lo = (G.key_nb[0] > 0 ? bucket_count_le(R, BR, G.key_shift[0], G.key_base[0],
+                                                          G.key_nb[0], radius, radius)
+                                         : count_le(R, nr, radius)) - 1;
         if (lo < 0) i = 0;
         else if (lo >= nr - 1) i = nr - 1;
         else i = (radius > R[nr + lo]) ? lo + 1 : lo;
@@ -561,7 +605,10 @@ GRT_HD bool locate_separable(const GridView& G, double radius, double ct, double
     // polar (scattering.hpp:67-81)
     int j;
     {
-        const int lo = count_ge_desc(T, nth, ct) - 1;
+        const int lo = (G.key_nb[1] > 0 ? bucket_count_ge_desc(T, BT, G.key_shift[1],
+                                                               G.key_base[1], G.key_nb[1],
+                                                               3.0 - ct, ct)
+                                         : count_ge_desc(T, nth, ct)) - 1;
         if (lo < 0) j = 0;
         else if (lo >= nth - 1) j = nth - 1;
         else j = (ct < T[nth + lo]) ? lo + 1 : lo;
@@ -571,7 +618,10 @@ GRT_HD bool locate_separable(const GridView& G, double radius, double ct, double
     int k = 0;
     double best = X * F[nph] + Y * F[2 * nph];
     {
-        const int lo = count_le(F, nph, pseudo_angle(X, Y)) - 1;
+        const double pa = pseudo_angle(X, Y);
+        const int lo = (G.key_nb[2] > 0 ? bucket_count_le(F, BF, G.key_shift[2], G.key_base[2],
+                                                          G.key_nb[2], pa + 4.0, pa)
+                                         : count_le(F, nph, pa)) - 1;
         const int c1 = lo, c2 = lo + 1, c3 = nph - 1;
         if (c1 > 0 && c1 < nph) {
             const double d = X * F[nph + c1] + Y * F[2 * nph + c1];

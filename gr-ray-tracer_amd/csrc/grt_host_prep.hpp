@@ -6,6 +6,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstring>
 #include <thread>
 #include <vector>
 
@@ -89,7 +91,57 @@ struct HostGrid {
     std::vector<double> tables;   // separable search tables, see locate_separable
     std::vector<double> axes;     // r_i | theta_j | phi_k of a separable grid
     std::vector<double> cells;    // kCellDoubles per cell
+    int key[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // key_shift[3], key_base[3], key_nb[3]
 };
+
+// Buckets per search axis at most (2-byte counts, appended to the tables).
+constexpr int kMaxBuckets = 1024;
+
+// Bucket counts of one search axis (axis_bucket on the device): `w` are the
+// keyed entries, non-decreasing; picks the finest key (most mantissa bits)
+// with at most kMaxBuckets buckets and appends, per bucket b = 0..nb, the
+// number of entries keyed below b.  Returns false (no buckets) when the
+// entries are not positive, finite and sorted.
+inline bool build_axis_buckets(const std::vector<double>& w, int& shift, int& base, int& nb,
+                               std::vector<uint16_t>& out)
+{
+    nb = 0;
+    const size_t n = w.size();
+    if (n == 0 || n > 65535) return false;
+    for (size_t j = 0; j < n; ++j)
+        if (!(w[j] > 0.0) || !std::isfinite(w[j]) || (j > 0 && !(w[j] >= w[j - 1]))) return false;
+    auto bits = [](double v) {
+        uint64_t u;
+        std::memcpy(&u, &v, sizeof u);
+        return u;
+    };
+    const uint64_t lo_bits = bits(w.front()), hi_bits = bits(w.back());
+    for (int mant = 20; mant >= 0; --mant) {
+        const int sh = 52 - mant;
+        const uint64_t span = (hi_bits >> sh) - (lo_bits >> sh) + 1;
+        if (span <= (uint64_t)kMaxBuckets) {
+            shift = sh;
+            base = (int)(lo_bits >> sh);
+            nb = (int)span;
+            break;
+        }
+    }
+    if (nb == 0) return false;
+    std::vector<int> key(n);
+    for (size_t j = 0; j < n; ++j) {
+        key[j] = axis_bucket(w[j], shift, base, nb);
+        if (j > 0 && key[j] < key[j - 1]) {
+            nb = 0;
+            return false;
+        }
+    }
+    size_t j = 0;
+    for (int b = 0; b <= nb; ++b) {
+        while (j < n && key[j] < b) ++j;
+        out.push_back((uint16_t)j);
+    }
+    return true;
+}
 
 // Largest separable table set staged in LDS (bytes).
 constexpr size_t kMaxTableBytes = 48 * 1024;
@@ -108,7 +160,8 @@ inline double host_pseudo_angle(double X, double Y)
 // Builds the separable search tables of locate_separable from the 1-D axes;
 // returns false if the axes do not admit the fast path.
 inline bool build_search_tables(const std::vector<double>& ra, const std::vector<double>& ta,
-                                const std::vector<double>& pa, std::vector<double>& tab)
+                                const std::vector<double>& pa, std::vector<double>& tab,
+                                int key[9])
 {
     const int nr = (int)ra.size(), nth = (int)ta.size(), nph = (int)pa.size();
     const double pi = 3.14159265358979323846, two_pi = 2.0 * pi;
@@ -165,6 +218,24 @@ inline bool build_search_tables(const std::vector<double>& ra, const std::vector
     }
     for (int k = 1; k < nph; ++k)
         if (!(F[k] > F[k - 1])) return false;
+    // bucket counts: radial on r, polar on 3 - cos theta (ascending), azimuth
+    // on pseudo-angle + 4; the device keys its query the same way
+    std::vector<double> wr(ra), wt(nth), wp(nph);
+    for (int j = 0; j < nth; ++j) wt[j] = 3.0 - T[j];
+    for (int k = 0; k < nph; ++k) wp[k] = F[k] + 4.0;
+    std::vector<uint16_t> counts;
+    const std::vector<double>* w[3] = {&wr, &wt, &wp};
+    for (int a = 0; a < 3; ++a) {
+        if (!build_axis_buckets(*w[a], key[a], key[3 + a], key[6 + a], counts)) {
+            for (int q = 0; q < 9; ++q) key[q] = 0;
+            counts.clear();
+            break;
+        }
+    }
+    const size_t head = tab.size();
+    tab.resize(head + (counts.size() * sizeof(uint16_t) + sizeof(double) - 1) / sizeof(double), 0.0);
+    if (!counts.empty())
+        std::memcpy(tab.data() + head, counts.data(), counts.size() * sizeof(uint16_t));
     return true;
 }
 
@@ -194,7 +265,8 @@ inline void prepare_grid(const grt_params& p, int nr, int nth, int nph, const do
     for (int k = 0; k < nph; ++k) pa[k] = at(phi, 0, 0, k);
     out.tables.clear();
     out.axes.clear();
-    if (sep) sep = build_search_tables(ra, ta, pa, out.tables);
+    for (int q = 0; q < 9; ++q) out.key[q] = 0;
+    if (sep) sep = build_search_tables(ra, ta, pa, out.tables, out.key);
     out.separable = sep;
     if (sep) {
         out.axes = ra;
@@ -238,9 +310,14 @@ inline void prepare_grid(const grt_params& p, int nr, int nth, int nph, const do
 // GridView over host (or device) copies of HostGrid's arrays.
 inline GridView grid_view(int nr, int nth, int nph, bool available, bool separable,
                           const double* tables, const double* axes, const double* coords3,
-                          const double* cells)
+                          const double* cells, const int* key)
 {
     GridView g{};
+    for (int a = 0; a < 3; ++a) {
+        g.key_shift[a] = key[a];
+        g.key_base[a] = key[3 + a];
+        g.key_nb[a] = (available && separable) ? key[6 + a] : 0;
+    }
     g.nr = nr;
     g.nth = nth;
     g.nph = nph;

@@ -39,7 +39,7 @@ void prepare(const grt_params* p, int nr, int nth, int nph, const double* r, con
         }
     }
     out.view = grid_view(nr, nth, nph, avail, avail && out.hg.separable, out.hg.tables.data(),
-                         out.hg.axes.data(), out.coords.data(), out.hg.cells.data());
+                         out.hg.axes.data(), out.coords.data(), out.hg.cells.data(), out.hg.key);
 }
 
 }  // namespace
@@ -67,6 +67,66 @@ int harness_locate(const grt_params* p, int nr, int nth, int nph, const double* 
         out[4 * q + 3] = k;
     }
     return pr.view.separable ? 1 : 0;
+}
+
+// The separable search on raw kernel inputs (radius, cos theta, X, Y), with
+// the bucket index (no_buckets = 0) or with plain binary searches over the
+// whole axes (no_buckets = 1).  Returns the number of buckets per axis in
+// nb_out[3] (0: the grid took no buckets).
+int harness_locate_raw(const grt_params* p, int nr, int nth, int nph, const double* r,
+                       const double* th, const double* ph, const double* rho, const double* tg,
+                       const double* vel, const double* mag, int no_buckets, int64_t nq,
+                       const double* radius, const double* ct, const double* X, const double* Y,
+                       int32_t* out, int32_t* nb_out)
+{
+    Prepared pr;
+    prepare(p, nr, nth, nph, r, th, ph, rho, tg, vel, mag, 0, pr);
+    for (int a = 0; a < 3; ++a) nb_out[a] = pr.view.key_nb[a];
+    if (!pr.view.separable) return 0;
+    if (no_buckets)
+        for (int a = 0; a < 3; ++a) pr.view.key_nb[a] = 0;
+    for (int64_t q = 0; q < nq; ++q) {
+        int i = -1, j = -1, k = -1;
+        const bool ok = locate_separable(pr.view, radius[q], ct[q], X[q], Y[q], i, j, k);
+        out[4 * q + 0] = ok ? 1 : 0;
+        out[4 * q + 1] = i;
+        out[4 * q + 2] = j;
+        out[4 * q + 3] = k;
+    }
+    return 1;
+}
+
+// Per axis (0 radial, 1 polar, 2 azimuth) and query, the plain count over
+// the axis table and the bucketed count: out[2 q] and out[2 q + 1].  The
+// queries are the kernel's compared values (r, cos theta, pseudo-angle).
+int harness_axis_counts(const grt_params* p, int nr, int nth, int nph, const double* r,
+                        const double* th, const double* ph, const double* rho, const double* tg,
+                        const double* vel, const double* mag, int axis, int64_t nq,
+                        const double* v, int32_t* out)
+{
+    Prepared pr;
+    prepare(p, nr, nth, nph, r, th, ph, rho, tg, vel, mag, 0, pr);
+    const GridView& G = pr.view;
+    if (!G.separable || G.key_nb[axis] == 0) return 0;
+    const double* R = G.tab;
+    const double* T = R + 4 * nr;
+    const double* F = T + 4 * nth;
+    const uint16_t* B = reinterpret_cast<const uint16_t*>(F + 4 * nph);
+    for (int a = 0; a < axis; ++a) B += G.key_nb[a] + 1;
+    for (int64_t q = 0; q < nq; ++q) {
+        const int sh = G.key_shift[axis], bs = G.key_base[axis], nb = G.key_nb[axis];
+        if (axis == 0) {
+            out[2 * q] = count_le(R, nr, v[q]);
+            out[2 * q + 1] = bucket_count_le(R, B, sh, bs, nb, v[q], v[q]);
+        } else if (axis == 1) {
+            out[2 * q] = count_ge_desc(T, nth, v[q]);
+            out[2 * q + 1] = bucket_count_ge_desc(T, B, sh, bs, nb, 3.0 - v[q], v[q]);
+        } else {
+            out[2 * q] = count_le(F, nph, v[q]);
+            out[2 * q + 1] = bucket_count_le(F, B, sh, bs, nb, v[q] + 4.0, v[q]);
+        }
+    }
+    return 1;
 }
 
 int harness_trace(const grt_params* p, int nr, int nth, int nph, const double* r,

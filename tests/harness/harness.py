@@ -16,8 +16,19 @@ LIB = os.path.join(HERE, "libgrt_host_harness.so")
 _lib = None
 
 
+def _sources():
+    csrc = os.path.join(REPO, "gr-ray-tracer_amd", "csrc")
+    return [os.path.join(HERE, f) for f in ("grt_host_harness.cpp", "grt_math_check.hip",
+                                             "build.sh")] + \
+        [os.path.join(csrc, f) for f in os.listdir(csrc) if f.endswith((".hpp", ".h"))] + \
+        [os.path.join(REPO, "include", f) for f in os.listdir(os.path.join(REPO, "include"))]
+
+
 def build(force=False):
-    if force or not os.path.exists(LIB):
+    """Builds the harness when it is missing or older than its sources."""
+    stale = not os.path.exists(LIB) or \
+        os.path.getmtime(LIB) < max(os.path.getmtime(f) for f in _sources())
+    if force or stale:
         subprocess.run(["sh", os.path.join(HERE, "build.sh")], check=True)
 
 
@@ -34,6 +45,12 @@ def lib():
         L.harness_locate.restype = C.c_int
         L.harness_locate.argtypes = [C.POINTER(grt_amd.Params), C.c_int, C.c_int, C.c_int] + \
             [vp] * 7 + [C.c_int, C.c_int64, vp, vp, vp, vp]
+        L.harness_axis_counts.restype = C.c_int
+        L.harness_axis_counts.argtypes = [C.POINTER(grt_amd.Params), C.c_int, C.c_int, C.c_int] + \
+            [vp] * 7 + [C.c_int, C.c_int64, vp, vp]
+        L.harness_locate_raw.restype = C.c_int
+        L.harness_locate_raw.argtypes = [C.POINTER(grt_amd.Params), C.c_int, C.c_int, C.c_int] + \
+            [vp] * 7 + [C.c_int, C.c_int64, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -79,3 +96,26 @@ def locate(params, fields, radius, polar, az, force_general=False):
                                radius.ctypes.data, polar.ctypes.data, az.ctypes.data,
                                out.ctypes.data)
     return out, bool(sep)
+
+
+def locate_raw(params, fields, radius, ct, X, Y, no_buckets=False):
+    """The separable search on the kernel's own inputs; returns (found, i, j, k)
+    rows and the bucket count of each axis."""
+    args, keep = _grid_args(fields)
+    q = [np.ascontiguousarray(v, dtype=np.float64) for v in (radius, ct, X, Y)]
+    out = np.zeros((len(q[0]), 4), dtype=np.int32)
+    nb = np.zeros(3, dtype=np.int32)
+    lib().harness_locate_raw(C.byref(params), *args, int(no_buckets), len(q[0]),
+                             *[v.ctypes.data for v in q], out.ctypes.data, nb.ctypes.data)
+    return out, nb
+
+
+def axis_counts(params, fields, axis, values):
+    """(plain, bucketed) counts of the separable search's axis tables for the
+    compared values (r, cos theta or pseudo-angle); None without buckets."""
+    args, keep = _grid_args(fields)
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    out = np.zeros((len(v), 2), dtype=np.int32)
+    ok = lib().harness_axis_counts(C.byref(params), *args, axis, len(v), v.ctypes.data,
+                                   out.ctypes.data)
+    return (out[:, 0], out[:, 1]) if ok else None

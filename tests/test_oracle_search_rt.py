@@ -129,6 +129,73 @@ def test_kernel_fast_search_matches_oracle(oracle):
         assert ties <= len(radius) - q
 
 
+def _keyed_neighbours(v):
+    """v with its two neighbouring doubles."""
+    return np.concatenate([v, np.nextafter(v, -np.inf), np.nextafter(v, np.inf)])
+
+
+def test_bucketed_search_equals_plain_binary_search():
+    """The bucket index only narrows the binary searches; with and without it
+    the separable search returns the same (found, i, j, k) for every query,
+    including queries exactly on table entries, midpoints and window edges,
+    on their neighbouring doubles, on bucket boundaries (key changes) and
+    outside the axes.  Grids: log-spaced radii over many octaves, uniform and
+    pole-clustered polar angles, and a single-azimuth grid."""
+    import grt_amd
+    import harness
+    rng = np.random.default_rng(11)
+    p = grt_amd.default_params(a_bh=0.0)
+    grids = [
+        (np.geomspace(1.2, 1.0e4, 300), np.linspace(0.01, np.pi - 0.01, 128),
+         np.linspace(0.0, 2 * np.pi, 64, endpoint=False)),
+        (np.geomspace(1.5, 100.0, 256), np.arccos(np.linspace(0.999, -0.999, 96)),
+         np.sort(rng.uniform(0.0, 2 * np.pi, 40))),
+        (np.sort(rng.uniform(2.0, 30.0, 17)), np.sort(rng.uniform(0.001, np.pi - 0.001, 9)),
+         np.array([0.7])),
+    ]
+    for ra, ta, pa in grids:
+        R, T, P = np.meshgrid(ra, ta, pa, indexing="ij")
+        f = dict(r=R, theta=T, phi=P, rho=np.ones(R.shape), Tgas=np.ones(R.shape),
+                 vel=np.zeros(R.shape + (4,)), mag=np.zeros(R.shape + (4,)))
+        # radial queries: entries, geometric midpoints, window-ish values,
+        # powers-of-two edges (bucket boundaries) and random values
+        rq = np.concatenate([ra, np.sqrt(ra[1:] * ra[:-1]), ra * np.exp(0.5 * np.log(ra[-1] / ra[0]) /
+                             len(ra)), 2.0 ** np.arange(0, 15), 2.0 ** np.arange(0, 15) * 1.5,
+                             np.exp(rng.uniform(np.log(0.5), np.log(3e4), 3000))])
+        rq = _keyed_neighbours(rq)
+        # polar queries in cos space: cos of entries and midpoints, +-1, random
+        cq = np.concatenate([np.cos(ta), np.cos(0.5 * (ta[1:] + ta[:-1])), [1.0, -1.0, 0.0],
+                             3.0 - 2.0 ** -np.arange(1, 12), rng.uniform(-1.0, 1.0, 3000)])
+        cq = np.clip(_keyed_neighbours(cq), -1.0, 1.0)
+        # azimuth: exact entry directions, midpoints, random
+        ang = np.concatenate([pa, 0.5 * (pa[1:] + pa[:-1]), rng.uniform(0, 2 * np.pi, 3000)])
+        X = np.concatenate([np.cos(ang), np.nextafter(np.cos(ang), 2.0)])
+        Y = np.concatenate([np.sin(ang), np.sin(ang)])
+        n = max(len(rq), len(cq), len(X))
+        rq, cq, X, Y = (np.resize(v, n) for v in (rq, cq, X, Y))
+        # pair every radial query with in-range polar/azimuth values too, so
+        # the later axes are reached
+        rq2 = np.resize(ra[rng.integers(0, len(ra), n)], n)
+        cq2 = np.resize(np.cos(ta[rng.integers(0, len(ta), n)]), n)
+        # the counts themselves, on every compared value and its neighbours
+        Xs, Ys = np.cos(ang), np.sin(ang)
+        pq = np.where(Ys >= 0, np.where(Xs >= 0, Ys / np.where(Xs + Ys > 0, Xs + Ys, 1.0),
+                                        1.0 - Xs / (Ys - Xs)),
+                      np.where(Xs < 0, 2.0 - Ys / (-Xs - Ys), 3.0 + Xs / (Xs - Ys)))
+        pq = _keyed_neighbours(np.concatenate([pq, [0.0, 1.0, 2.0, 3.0, 4.0 - 2.0 ** -50]]))
+        for axis, vals in ((0, rq), (1, cq), (2, pq)):
+            plain, keyed = harness.axis_counts(p, f, axis, vals)
+            assert np.array_equal(plain, keyed), axis
+        for args in ((rq, cq, X, Y), (rq2, cq, X, Y), (rq2, cq2, X, Y), (rq, cq2, X, Y)):
+            a, nb = harness.locate_raw(p, f, *args, no_buckets=False)
+            b, nb0 = harness.locate_raw(p, f, *args, no_buckets=True)
+            assert (nb > 0).all() and (nb <= 1024).all(), nb
+            assert np.array_equal(a[:, 0], b[:, 0])
+            found = a[:, 0] == 1
+            assert found.any()
+            assert np.array_equal(a[found], b[found])
+
+
 def test_kernel_general_search_matches_oracle(oracle):
     import grt_amd
     import harness
