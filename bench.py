@@ -42,9 +42,9 @@ PEAK_FP64_TFLOPS = 78.6     # MI355X vector FP64, published
 
 # HBM bytes per trace_fp64 launch of the default workload, from two
 # rocprofv3 --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, the factors a
-# plain 512 MiB device copy reads on the same counters); the profile summary
+# plain 1 GiB device copy reads on the same counters); the profile summary
 # profiles/trace_fp64_rk45_fm_torus_256x128x128_1024px.md holds the commands.
-TRAFFIC_BYTES_DEFAULT = 942.3e6
+TRAFFIC_BYTES_DEFAULT = 398.9e6
 
 
 def workload(args):
