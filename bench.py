@@ -342,7 +342,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(photons would shard by id with one image all-reduce, see DESIGN.md)",
+              "(photons would shard by id, with one all-reduce of the image sums)",
               file=sys.stderr)
         sys.exit(2)
 
