@@ -497,7 +497,7 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
     GRT_HIP(c, hipMemcpyAsync(c->d_consts, &c->h_consts, sizeof c->h_consts,
                               hipMemcpyHostToDevice, stream));
     grt::TraceLaunch L{};
-    L.c = c->h_consts;
+    L.consts = c->d_consts;
     L.integrator = P.integrator;
     L.grid = make_grid_view(c);
     L.queue = c->d_counters + 0;

@@ -37,9 +37,9 @@ enum StatSlot {
     kStatCount
 };
 
-// Launch constants.  The tracing kernels take them by value in their
-// argument block (constant memory: scalar loads, no wait on the kernel's own
-// photon stores); the adjoint reads the device copy the last trace left.
+// Launch constants, in device memory.  The tracing kernels read them through
+// a constant-address-space pointer (scalar loads, no wait on the kernel's
+// own photon stores); the adjoint reads the copy the last trace left.
 struct TraceConsts {
     PhotonPtrs ph;
     StepParams P;
@@ -74,7 +74,7 @@ inline size_t trace_lds_bytes(int lds_doubles, int block)
 }
 
 struct TraceLaunch {
-    TraceConsts c;
+    const TraceConsts* consts;
     GridView grid;
     int integrator;
     const uint32_t* list_in;  // nullptr: the identity list (slot i is photon i)

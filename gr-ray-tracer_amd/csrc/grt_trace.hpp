@@ -40,6 +40,14 @@ __device__ __forceinline__ uint32_t wave_claim(uint32_t* counter)
     return base + rank;
 }
 
+// The photon arrays live in global memory; saying so keeps their accesses
+// global (vector memory only) whatever the pointer was loaded from.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p)
+{
+    return (__attribute__((address_space(1))) T*)p;
+}
+
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
 {
 #pragma unroll
@@ -60,16 +68,16 @@ __device__ __forceinline__ void stage_fill(WaveStage* S, const TraceLaunch& a,
     const uint32_t i = a.list_in ? a.list_in[slot] : slot;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        S->v[q][l] = p.x[q][i];
-        S->v[4 + q][l] = p.k[q][i];
-        S->v[8 + q][l] = p.st[q][i];
+        S->v[q][l] = gmem(p.x[q])[i];
+        S->v[4 + q][l] = gmem(p.k[q])[i];
+        S->v[8 + q][l] = gmem(p.st[q])[i];
     }
-    S->v[12][l] = p.freq[i];
-    S->v[13][l] = p.eem[i];
-    S->v[14][l] = p.dl[i];
-    S->id[l] = p.id[i];
-    S->draw[l] = p.rng_draw[i];
-    S->phase[l] = p.phase[i];
+    S->v[12][l] = gmem(p.freq)[i];
+    S->v[13][l] = gmem(p.eem)[i];
+    S->v[14][l] = gmem(p.dl)[i];
+    S->id[l] = gmem(p.id)[i];
+    S->draw[l] = gmem(p.rng_draw)[i];
+    S->phase[l] = gmem(p.phase)[i];
     S->idx[l] = i;
 }
 
@@ -139,20 +147,20 @@ __device__ __forceinline__ void store_photon(const PhotonPtrs& p, uint32_t idx, 
 {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        p.x[q][idx] = ph.s[q];
-        p.k[q][idx] = ph.s[4 + q];
-        p.st[q][idx] = ph.stokes[q];
+        gmem(p.x[q])[idx] = ph.s[q];
+        gmem(p.k[q])[idx] = ph.s[4 + q];
+        gmem(p.st[q])[idx] = ph.stokes[q];
     }
-    p.freq[idx] = ph.freq;
-    p.eem[idx] = ph.eem;
-    p.dl[idx] = ph.dl;
-    p.phase[idx] = ph.phase;
-    p.term[idx] = ph.term;
+    gmem(p.freq)[idx] = ph.freq;
+    gmem(p.eem)[idx] = ph.eem;
+    gmem(p.dl)[idx] = ph.dl;
+    gmem(p.phase)[idx] = ph.phase;
+    gmem(p.term)[idx] = ph.term;
     if (ph.phase == kArrived) {
-        p.udisp[idx] = au;
-        p.vdisp[idx] = av;
+        gmem(p.udisp)[idx] = au;
+        gmem(p.vdisp)[idx] = av;
     }
-    p.rng_draw[idx] = rng.draw;
+    gmem(p.rng_draw)[idx] = rng.draw;
 }
 
 #ifndef GRT_TRACE_WAVES_PER_EU
@@ -246,8 +254,15 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a)
         __syncthreads();
         G.tab = lds;
     }
-    const StepParams& P = a.c.P;
-    const PhotonPtrs& PH = a.c.ph;
+    // The launch constants (step parameters, photon array pointers) are read
+    // through a constant-address-space pointer that the loop re-derives
+    // opaquely every iteration: scalar loads at their use sites, instead of
+    // some 130 scalar registers held across the whole loop and spilled to
+    // vector lanes.
+    typedef __attribute__((address_space(4))) const TraceConsts ConstTC;
+    const ConstTC* c0 = (const ConstTC*)a.consts;
+    const StepParams& P = *(const StepParams*)&c0->P;
+    const PhotonPtrs& PH = *(const PhotonPtrs*)&c0->ph;
 
     // This wavefront's photon stage follows the tables.
     WaveStage* S = reinterpret_cast<WaveStage*>(lds + ((a.lds_doubles + 1) & ~1)) +
@@ -273,6 +288,10 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a)
     // without one sit out the step and join the refills, which need the
     // whole wavefront.
     while (__ballot(have) != 0) {
+        const ConstTC* cp = c0;
+        asm volatile("" : "+s"(cp));
+        const StepParams& P = *(const StepParams*)&cp->P;
+        const PhotonPtrs& PH = *(const PhotonPtrs*)&cp->ph;
         bool need = false;
         if (have) {
             double au = 0.0, av = 0.0;
