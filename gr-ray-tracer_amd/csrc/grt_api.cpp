@@ -494,6 +494,7 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
 
     c->h_consts.ph = to_ptrs(*dev, c->d_rng);
     c->h_consts.P = P;
+    c->h_consts.grid = make_grid_view(c);
     GRT_HIP(c, hipMemcpyAsync(c->d_consts, &c->h_consts, sizeof c->h_consts,
                               hipMemcpyHostToDevice, stream));
     grt::TraceLaunch L{};

@@ -43,6 +43,7 @@ enum StatSlot {
 struct TraceConsts {
     PhotonPtrs ph;
     StepParams P;
+    GridView grid;   // the launch's grid (the loop swaps in the LDS tables)
 };
 
 // One wavefront's photon stage in LDS (grt_trace.hpp): the photons of the

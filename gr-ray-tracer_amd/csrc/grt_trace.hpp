@@ -254,7 +254,7 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a)
         __syncthreads();
         G.tab = lds;
     }
-    // The launch constants (step parameters, photon array pointers) are read
+    // The launch constants (step parameters, photon array pointers, grid) are read
     // through a constant-address-space pointer that the loop re-derives
     // opaquely every iteration: scalar loads at their use sites, instead of
     // some 130 scalar registers held across the whole loop and spilled to
@@ -292,6 +292,8 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a)
         asm volatile("" : "+s"(cp));
         const StepParams& P = *(const StepParams*)&cp->P;
         const PhotonPtrs& PH = *(const PhotonPtrs*)&cp->ph;
+        GridView G = *(const GridView*)&cp->grid;
+        if (kSep) G.tab = lds;
         bool need = false;
         if (have) {
             double au = 0.0, av = 0.0;

@@ -11,17 +11,16 @@ photon geodesics equal the oracle's bit for bit (the taped trace is the FP64
 kernel) and the Stokes parameters agree to 1e-13 relative (the device exp()
 behind exp(-dtau) and glibc's may differ in the last bit); gradients agree
 with the oracle's to 1e-10 relative (summation order of the atomics aside);
-central differences (relative step 1e-6) agree to 1e-5 relative.
+central differences agree to 1e-5 relative (relative step 1e-4, on the cells
+whose difference L resolves).
 """
 import numpy as np
 import pytest
 
 from conftest import c1_params
-from test_oracle_adjoint import small_torus, disk_setup
+from test_oracle_adjoint import H_CELL, disk_setup, fd_cells, small_torus
 
 pytestmark = pytest.mark.gpu
-
-H = 1e-6
 
 
 def gpu_setup(kw, fields):
@@ -88,17 +87,19 @@ def test_adjoint_matches_gpu_finite_differences(oracle, gpu):
     gr, gt, gc = ctx.adjoint(w)
     gr, gt = gr.cpu().numpy(), gt.cpu().numpy()
     rho0, t0 = fields["rho"], fields["Tgas"]
-    order = np.argsort(-np.abs(gr.ravel()))
-    for c in list(order[:4]) + list(order[40:44]):
-        for name, grad in (("rho", gr), ("Tgas", gt)):
-            base = {"rho": rho0, "Tgas": t0}
+    base = {"rho": rho0, "Tgas": t0}
+    L0 = L(rho0, t0)
+    rng = np.random.default_rng(6)
+    for name, grad in (("rho", gr), ("Tgas", gt)):
+        for c in fd_cells(grad, base[name], L0, rng):
             v0 = base[name].ravel()[c]
             plus = {k: a.copy() for k, a in base.items()}
             minus = {k: a.copy() for k, a in base.items()}
-            plus[name].ravel()[c] = v0 * (1 + H)
-            minus[name].ravel()[c] = v0 * (1 - H)
-            fd = (L(plus["rho"], plus["Tgas"]) - L(minus["rho"], minus["Tgas"])) / (2 * H * v0)
-            assert grad.ravel()[c] == pytest.approx(fd, rel=1e-5, abs=1e-12), (name, c)
+            plus[name].ravel()[c] = v0 * (1 + H_CELL)
+            minus[name].ravel()[c] = v0 * (1 - H_CELL)
+            fd = (L(plus["rho"], plus["Tgas"]) - L(minus["rho"], minus["Tgas"])) / \
+                (2 * H_CELL * v0)
+            assert grad.ravel()[c] == pytest.approx(fd, rel=1e-5), (name, c)
     ctx.set_fluid_scalars(rho0, t0)
 
 

@@ -120,6 +120,37 @@ def test_c1_image_64(oracle, gpu, c1):
         assert st_g[k] == st_o[k], k
 
 
+def test_work_list_edges(oracle, gpu, c1):
+    """The persistent loop's work list at its edges: batches smaller than,
+    equal to and just above one wavefront, photons already terminated on
+    entry (so the first list is not the identity), and tiny step budgets
+    that split every trace over many compacting launches, under several
+    launch shapes.  Every photon must end exactly as in the oracle."""
+    kw = c1_params()
+    su, sv = pixel_screen(16)
+    ctx = gpu_ctx(kw, c1)
+    po, G = oracle_parts(oracle, kw, c1)
+    full = oracle.init_image(po, su, sv, 0.1)
+    rng = np.random.default_rng(12)
+    for n in (1, 3, 63, 64, 65, 130, 256):
+        pick = np.sort(rng.choice(len(full["x0"]), n, replace=False))
+        ph0 = {k: np.ascontiguousarray(v[pick]) for k, v in full.items()}
+        if n > 3:
+            ph0["terminate"][rng.choice(n, n // 5, replace=False)] = 1
+        for tune in ({"block": 64, "blocks_per_cu": 1, "sub_budget": 3},
+                     {"block": 256, "blocks_per_cu": 2, "sub_budget": 1048576},
+                     {"block": 128, "blocks_per_cu": 4, "sub_budget": 7}):
+            ctx.tuning = tune
+            g, o, st_g, st_o = trace_both(oracle, ctx, po, G, ph0)
+            check_states(g, o, frac=1.0)
+            for k in ("x0", "x1", "k1", "I", "dlambda"):
+                assert np.array_equal(g[k], o[k]), (n, tune, k)
+            for k in ("functor_calls", "accepted_steps", "rejected_attempts", "global_steps",
+                      "terminated"):
+                assert st_g[k] == st_o[k], (n, tune, k)
+    ctx.tuning = {"block": 256, "blocks_per_cu": 2, "sub_budget": 256}
+
+
 def test_rk4_general_grid_and_capture(oracle, gpu):
     """RK4, the non-separable search path and horizon capture together."""
     from grt_amd.fm_torus import fishbone_moncrief
