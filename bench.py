@@ -322,6 +322,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default launch shape instead of autotuning it first")
+    ap.add_argument("--tune", default=None, metavar="BLOCK,PER_CU,BUDGET",
+                    help="use this launch shape for the headline render (no autotuning), "
+                         "e.g. to profile the shape an autotuned run picked")
     ap.add_argument("--no-precision", action="store_true",
                     help="skip the FP64-vs-mixed comparison on the 2048^2 a=0.94 image")
     ap.add_argument("--precision-image", type=int, default=2048)
@@ -385,7 +388,10 @@ def main():
     log(f"{len(batches)} batches of {n} rays resident; setup {time.perf_counter() - t_setup:.1f}s")
 
     tuning_table = None
-    if args.autotune:
+    if args.tune:
+        b, per_cu, budget = (int(x) for x in args.tune.split(","))
+        ctx.tuning = dict(block=b, blocks_per_cu=per_cu, sub_budget=budget)
+    elif args.autotune:
         from grt_amd.autotune import autotune
         best, tuning_table = autotune(ctx, proto, args.max_steps)
         log(f"autotuned launch shape {best}: " +
