@@ -261,6 +261,7 @@ int stage_photons(grt_ctx* c, uint64_t n, grt_photons* d)
 int copy_photons(grt_ctx* c, const grt_photons* dst, const grt_photons* src, uint64_t n,
                  hipMemcpyKind kind)
 {
+    if (n == 0) return GRT_OK;
     GRT_HIP(c, hipMemcpy(dst->id, src->id, n * 8, kind));
     const std::pair<double*, double*> dd[] = {
         {dst->frequency, src->frequency}, {dst->emission_frame_energy, src->emission_frame_energy},
@@ -465,7 +466,7 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
                grt_stats* stats, int tape_slots)
 {
     if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
-    if (!dev || !photons_complete(dev)) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
+    if (!dev || (n > 0 && !photons_complete(dev))) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
     if (n > 0x80000000ull) return fail(c, GRT_E_INVALID, "more than 2^31 photons in one call");
     if (max_steps <= 0) return fail(c, GRT_E_INVALID, "integration.max_steps must be positive");
     if (!(termination_percent > 0.0) || termination_percent > 1.0)
@@ -756,7 +757,7 @@ int grt_trace(grt_ctx* c, const grt_photons* host, uint64_t n, int32_t max_steps
               double termination_percent, int64_t output_interval, grt_stats* stats)
 {
     if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
-    if (!host || !photons_complete(host)) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
+    if (!host || (n > 0 && !photons_complete(host))) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
     GRT_HIP(c, hipSetDevice(c->device));
     grt_photons d{};
     int rc = stage_photons(c, n, &d);
@@ -803,7 +804,7 @@ int grt_init_image_device(grt_ctx* c, const grt_photons* dev, uint64_t n, uint64
                           void* stream_v)
 {
     if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
-    if (!dev || !photons_complete(dev)) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
+    if (!dev || (n > 0 && !photons_complete(dev))) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
     if ((screen_u == nullptr) != (screen_v == nullptr))
         return fail(c, GRT_E_INVALID, "screen_u and screen_v must both be given or both be NULL");
     if (!std::isfinite(initial_step)) return fail(c, GRT_E_INVALID, "initial_step must be finite");
@@ -821,7 +822,7 @@ int grt_init_image(grt_ctx* c, const grt_photons* host, uint64_t n, uint64_t id_
                    double initial_step, const double* screen_u, const double* screen_v)
 {
     if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
-    if (!host || !photons_complete(host)) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
+    if (!host || (n > 0 && !photons_complete(host))) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
     GRT_HIP(c, hipSetDevice(c->device));
     grt_photons d{};
     int rc = stage_photons(c, n, &d);
@@ -887,7 +888,7 @@ int grt_init_disk_device(grt_ctx* c, const grt_photons* dev, uint64_t n, uint64_
                          double initial_step, void* stream_v)
 {
     if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
-    if (!dev || !photons_complete(dev)) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
+    if (!dev || (n > 0 && !photons_complete(dev))) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
     if (!std::isfinite(initial_step)) return fail(c, GRT_E_INVALID, "initial_step must be finite");
     hipStream_t stream = static_cast<hipStream_t>(stream_v);
     GRT_HIP(c, hipSetDevice(c->device));
@@ -908,7 +909,7 @@ int grt_init_disk(grt_ctx* c, const grt_photons* host, uint64_t n, uint64_t id_o
                   double initial_step)
 {
     if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
-    if (!host || !photons_complete(host)) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
+    if (!host || (n > 0 && !photons_complete(host))) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
     GRT_HIP(c, hipSetDevice(c->device));
     grt_photons d{};
     int rc = stage_photons(c, n, &d);
@@ -924,7 +925,7 @@ int grt_image_products_device(grt_ctx* c, const grt_photons* dev, uint64_t n, in
                               double* spec_nu, double* spec_I, uint64_t* arrived, void* stream_v)
 {
     if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
-    if (!dev || !photons_complete(dev)) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
+    if (!dev || (n > 0 && !photons_complete(dev))) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
     if (nx <= 0 || ny <= 0 || bins <= 0)
         return fail(c, GRT_E_INVALID, "output image dimensions and spectrum_bins must be positive");
     if (!(smin > 0.0) || !(smax > smin))
@@ -959,7 +960,7 @@ int grt_image_products(grt_ctx* c, const grt_photons* host, uint64_t n, int64_t 
                        double* spec_I, uint64_t* arrived)
 {
     if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
-    if (!host || !photons_complete(host)) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
+    if (!host || (n > 0 && !photons_complete(host))) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
     if (nx <= 0 || ny <= 0 || bins <= 0)
         return fail(c, GRT_E_INVALID, "output image dimensions and spectrum_bins must be positive");
     if (!image || !spec_nu || !spec_I || !arrived)

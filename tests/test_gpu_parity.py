@@ -151,6 +151,30 @@ def test_work_list_edges(oracle, gpu, c1):
     ctx.tuning = {"block": 256, "blocks_per_cu": 2, "sub_budget": 256}
 
 
+def test_empty_batches(gpu, c1):
+    """Zero photons: every entry point succeeds and does nothing (the
+    reference's loops run over an empty photon view)."""
+    import grt_amd
+    ctx = gpu_ctx(c1_params(), c1)
+    for dev in (None, gpu):
+        ph = grt_amd.empty_photons(0, device=dev)
+        st = ctx.trace(ph, 100)
+        assert st["functor_calls"] == 0 and st["accepted_steps"] == 0 and st["terminated"] == 0
+        ctx.init_image(ph, 0.1)
+        img = ctx.image_products(ph, 8, 8)
+        assert img["arrived_count"] == 0 and not np.asarray(
+            img["image"].cpu() if hasattr(img["image"], "cpu") else img["image"]).any()
+    # more than 2^31 photons in one call is refused before any array is read
+    import ctypes as C
+    small = grt_amd.empty_photons(4, device=gpu)
+    s = grt_amd.photon_struct(small)
+    st = grt_amd.Stats()
+    rc = ctx._L.grt_trace_device(ctx._h, C.byref(s), (1 << 31) + 1, 10, 1.0, 0, None,
+                                 C.byref(st))
+    assert rc == -1                                   # GRT_E_INVALID
+    assert "2^31" in ctx._L.grt_last_error(ctx._h).decode()
+
+
 def test_rk4_general_grid_and_capture(oracle, gpu):
     """RK4, the non-separable search path and horizon capture together."""
     from grt_amd.fm_torus import fishbone_moncrief
