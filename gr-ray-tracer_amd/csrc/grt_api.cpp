@@ -50,6 +50,8 @@ struct grt_ctx {
     uint32_t* d_lists = nullptr;  // two lists of list_cap entries
     size_t list_cap = 0;
     uint32_t* d_rng = nullptr;
+    uint8_t* d_fpend = nullptr;     // deferred thermal frequencies (rng_cap entries)
+    double* d_fratio = nullptr;
     size_t rng_cap = 0;
     uint32_t* d_counters = nullptr;  // [0] queue, [1] n_out, [2] live count
     unsigned long long* d_stats = nullptr;
@@ -193,10 +195,16 @@ int ensure_scratch(grt_ctx* c, uint64_t n)
     }
     if (n > c->rng_cap) {
         if (c->d_rng) (void)hipFree(c->d_rng);
+        if (c->d_fpend) (void)hipFree(c->d_fpend);
+        if (c->d_fratio) (void)hipFree(c->d_fratio);
         c->d_rng = nullptr;
+        c->d_fpend = nullptr;
+        c->d_fratio = nullptr;
         c->rng_cap = 0;
         const size_t cap = std::max<uint64_t>(n, 1024);
         GRT_HIP(c, hipMalloc(&c->d_rng, cap * sizeof(uint32_t)));
+        GRT_HIP(c, hipMalloc(&c->d_fpend, cap * sizeof(uint8_t)));
+        GRT_HIP(c, hipMalloc(&c->d_fratio, cap * sizeof(double)));
         c->rng_cap = cap;
     }
     return GRT_OK;
@@ -355,8 +363,8 @@ int grt_destroy(grt_ctx* c)
 {
     if (!c) return GRT_OK;
     free_grid(c);
-    void* bufs[] = {c->d_lists, c->d_rng, c->d_counters, c->d_stats, c->d_stage, c->d_img,
-                    c->d_consts, c->d_tape, c->d_thdr, c->d_rec};
+    void* bufs[] = {c->d_lists, c->d_rng, c->d_fpend, c->d_fratio, c->d_counters, c->d_stats,
+                    c->d_stage, c->d_img, c->d_consts, c->d_tape, c->d_thdr, c->d_rec};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -485,6 +493,7 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
 
     grt_stats st{};
     GRT_HIP(c, hipMemsetAsync(c->d_rng, 0, n * sizeof(uint32_t), stream));
+    GRT_HIP(c, hipMemsetAsync(c->d_fpend, 0, n * sizeof(uint8_t), stream));
     GRT_HIP(c, hipMemsetAsync(c->d_stats, 0, grt::kStatCount * sizeof(unsigned long long), stream));
     GRT_HIP(c, hipMemsetAsync(c->d_counters, 0, 8 * sizeof(uint32_t), stream));
     uint32_t* list[2] = {c->d_lists, c->d_lists + c->list_cap};
@@ -494,6 +503,8 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
     GRT_HIP(c, hipStreamSynchronize(stream));
 
     c->h_consts.ph = to_ptrs(*dev, c->d_rng);
+    c->h_consts.ph.fpend = c->d_fpend;
+    c->h_consts.ph.fratio = c->d_fratio;
     c->h_consts.P = P;
     c->h_consts.grid = make_grid_view(c);
     GRT_HIP(c, hipMemcpyAsync(c->d_consts, &c->h_consts, sizeof c->h_consts,
@@ -594,6 +605,16 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
         g = check + 1;
         if (fraction >= termination_percent) break;
         if (!exact_schedule && live == 0) break;
+    }
+    if (n > 0) {
+        // Thermal frequencies whose draw the tracing kernel deferred.
+        GRT_HIP(c, hipEventRecord(c->ev0, stream));
+        GRT_HIP(c, grt::launch_resolve_deferred(c->h_consts.ph, n, P.seed, P.stream << 1, stream));
+        GRT_HIP(c, hipEventRecord(c->ev1, stream));
+        GRT_HIP(c, hipStreamSynchronize(stream));
+        float ms = 0.0f;
+        GRT_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        kernel_ms += ms;
     }
     if (tape_slots > 0) {
         GRT_HIP(c, grt::launch_tape_finalize(c->d_thdr, n, tape_slots, c->d_counters + 3, stream));

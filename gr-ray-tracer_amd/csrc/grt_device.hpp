@@ -891,6 +891,29 @@ struct PhotonRng {
 // Emission, scattering and camera
 // ---------------------------------------------------------------------------
 
+// Frequency of a thermal packet at temperature T (GenerateBlackbody,
+// photon_generation.hpp:99-127): five draws, always.
+GRT_HD double thermal_frequency(double T, PhotonRng& rng)
+{
+    const double inverse_zeta4 = 0.9239384029215902;
+    const double choose = rng.drand();
+    double cum = 0.0;
+    int idx = 1;
+    for (; idx < 64; ++idx) {
+        const double n = (double)idx;
+        cum += inverse_zeta4 / (n * n * n * n);
+        if (choose <= cum) break;
+    }
+    const double d1 = rng.drand();
+    const double d2 = rng.drand();
+    const double d3 = rng.drand();
+    const double d4 = rng.drand();
+    const double prod = fmax(1.0e-300, d1 * d2 * d3 * d4);
+    const double xx = -log(prod) / (double)idx;
+    return xx * 2.0836619123327576e10 * T;
+}
+constexpr uint32_t kThermalDraws = 5;
+
 // GenerateBlackbody / GeneratePowerLaw (photon_generation.hpp:99-143), the
 // GeneratePhotonsCustom plug-in, selected as integrate_cart_ks.hpp:332-347.
 GRT_HD bool generate_photon(const StepParams& P, const FluidCellCGS& cell, PhotonRng& rng,
@@ -898,22 +921,7 @@ GRT_HD bool generate_photon(const StepParams& P, const FluidCellCGS& cell, Photo
 {
     if (P.generator == 0) {
         if (!(cell.temperature_k > 0.0)) return false;
-        const double inverse_zeta4 = 0.9239384029215902;
-        const double choose = rng.drand();
-        double cum = 0.0;
-        int idx = 1;
-        for (; idx < 64; ++idx) {
-            const double n = (double)idx;
-            cum += inverse_zeta4 / (n * n * n * n);
-            if (choose <= cum) break;
-        }
-        const double d1 = rng.drand();
-        const double d2 = rng.drand();
-        const double d3 = rng.drand();
-        const double d4 = rng.drand();
-        const double prod = fmax(1.0e-300, d1 * d2 * d3 * d4);
-        const double xx = -log(prod) / (double)idx;
-        nu = xx * 2.0836619123327576e10 * cell.temperature_k;
+        nu = thermal_frequency(cell.temperature_k, rng);
         energy = P.e_cell / (double)packets;
         return is_finite(nu) && nu > 0.0;
     }
@@ -1011,7 +1019,34 @@ struct Photon {
     double stokes[4];
     double freq, eem, dl;
     uint8_t phase, term;
+    uint8_t pend;   // thermal frequency deferred: freq holds -T (see post_step)
 };
+
+// Deferred thermal frequency: the frequency of the packet emitted at
+// temperature T = -ph.freq, drawn from the five draws reserved before
+// rng.draw (nothing draws in between: deferral is off with scattering).
+GRT_HD void resolve_frequency(Photon& ph, const PhotonRng& rng)
+{
+    PhotonRng r = rng;
+    r.draw -= kThermalDraws;
+    ph.freq = thermal_frequency(-ph.freq, r);
+    ph.pend = 0;
+}
+
+// After the trace: the deferred frequency, times the arrival redshift fr
+// (ph.freq *= eobs / eem in the immediate form; fr = 1 where arrival left
+// the frequency unchanged) for a photon that arrived.
+GRT_HD void finish_deferred(Photon& ph, const PhotonRng& rng, double fr)
+{
+    resolve_frequency(ph, rng);
+    if (ph.phase == kArrived) ph.freq *= fr;
+}
+
+// Between these temperatures a thermal frequency is always finite and
+// positive (the sampled x lies in [7e-18, 691]), so a deferred emission
+// succeeds exactly when the immediate one would.
+constexpr double kDeferTmin = 1.0e-280;
+constexpr double kDeferTmax = 1.0e290;
 
 // Restores g^{mu nu} p_mu p_nu = 0 (mixed-precision form, after a scattering
 // event) by scaling the spatial momentum (p_t, the conserved energy, is kept): one
@@ -1046,9 +1081,18 @@ GRT_HD void null_reproject(double s[8], const StepParams& P)
 // the attenuation factor exp(-dtau) in FP32; neither feeds back into the
 // geodesic or its step-size sequence.  A photon scattered with the FP32
 // tetrad is put back on the null cone in FP64 (null_reproject).
-template <int kSep, int kMixed = 0, class Hook = NoHook>
+//
+// kDefer: the image-mode emission of a thermal packet (generator 0, no
+// scattering) only reserves its five draws and keeps -T in freq (pend set);
+// the frequency is drawn when a transfer update first needs it, or after the
+// trace (resolve_deferred), where every photon draws it at once instead of
+// one or two lanes at a time.  On arrival the redshift factor eobs / eem is
+// returned in fr for the deferred photon.  Results are bit for bit those of
+// the immediate emission.
+template <int kSep, int kMixed = 0, class Hook = NoHook, int kDefer = 0>
 GRT_HD bool post_step(Photon& ph, uint8_t phase0, double taken, const GridView& G,
-                      const StepParams& P, PhotonRng& rng, double& au, double& av, Hook& hook)
+                      const StepParams& P, PhotonRng& rng, double& au, double& av, double& fr,
+                      Hook& hook)
 {
     typedef typename std::conditional<kMixed != 0, float, double>::type T;
     const bool image = (P.mode == 0);
@@ -1064,9 +1108,21 @@ GRT_HD bool post_step(Photon& ph, uint8_t phase0, double taken, const GridView& 
         bool generated = false;
         const double dcode = fl.density_g_cm3 / P.density_scale;
         const int packets = ComputeSuperphotonsPerCell(dcode, P.norm);
-        if (packets > 0) generated = generate_photon(P, fl, rng, packets, nu, energy);
+        bool defer = false;
+        if (packets > 0) {
+            if (kDefer && P.generator == 0 && !P.scatter && fl.temperature_k > kDeferTmin &&
+                fl.temperature_k < kDeferTmax) {
+                nu = -fl.temperature_k;
+                energy = P.e_cell / (double)packets;
+                rng.draw += kThermalDraws;
+                generated = defer = true;
+            } else {
+                generated = generate_photon(P, fl, rng, packets, nu, energy);
+            }
+        }
         if (generated) {
             ph.freq = nu;
+            ph.pend = defer ? 1 : 0;
             double eem = 0.0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) eem -= ph.s[4 + q] * (double)e[q];
@@ -1082,6 +1138,7 @@ GRT_HD bool post_step(Photon& ph, uint8_t phase0, double taken, const GridView& 
         }
         hook.reset();
     } else if (sampled && phase0 != kBackward) {
+        if (kDefer && ph.pend) resolve_frequency(ph, rng);
         if (P.absorb || P.emit) hook.rt(fl, cell, ph.freq, taken, ph.stokes[0]);
         // apply_radiative_transfer_step, radiative_transfer.hpp:81-96
         if (P.absorb) {
@@ -1102,10 +1159,14 @@ GRT_HD bool post_step(Photon& ph, uint8_t phase0, double taken, const GridView& 
             const double eobs = observer_energy(P, ph.s);
             const double eem = ph.eem;
             if (eobs > 0.0 && eem > 0.0 && is_finite(eobs) && is_finite(eem)) {
-                ph.freq *= eobs / eem;
+                if (kDefer && ph.pend)
+                    fr = eobs / eem;
+                else
+                    ph.freq *= eobs / eem;
             } else {
                 ph.stokes[0] = ph.stokes[1] = ph.stokes[2] = ph.stokes[3] = 0.0;
                 hook.reset();
+                if (kDefer && ph.pend) fr = 1.0;
             }
             ph.phase = kArrived;
             ph.term = 1;
@@ -1227,9 +1288,10 @@ GRT_HD void lane_reset(Lane& L)
 // is thereby split at its retry loop, so lanes whose step is rejected retry
 // while their neighbours go on.  Returns true when the functor call
 // completed in this iteration; (au, av) are set when the photon arrived.
-template <int kIntegrator, int kSep, class Rhs = RhsExact, int kMixed = 0, class Hook = NoHook>
+template <int kIntegrator, int kSep, class Rhs = RhsExact, int kMixed = 0, class Hook = NoHook,
+          int kDefer = 0>
 GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, PhotonRng& rng,
-                         LaneCounters& n, double& au, double& av, Hook& hook)
+                         LaneCounters& n, double& au, double& av, double& fr, Hook& hook)
 {
     Photon& ph = L.ph;
     if (!L.in_call) {
@@ -1240,7 +1302,7 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
             const double taken = ph.dl;
             rk4_step<Rhs>(ph.s, taken, P);
             ++n.accepted;
-            n.samples += post_step<kSep, kMixed>(ph, L.phase0, taken, G, P, rng, au, av, hook) ? 1u : 0u;
+            n.samples += post_step<kSep, kMixed, Hook, kDefer>(ph, L.phase0, taken, G, P, rng, au, av, fr, hook) ? 1u : 0u;
             return true;
         }
         Rhs::eval(ph.s, P, L.k1);
@@ -1258,7 +1320,7 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
         // register allocator need not keep it alive across the post-step.
 #pragma unroll
         for (int q = 0; q < 8; ++q) L.k1[q] = 0.0;
-        n.samples += post_step<kSep, kMixed>(ph, L.phase0, taken, G, P, rng, au, av, hook) ? 1u : 0u;
+        n.samples += post_step<kSep, kMixed, Hook, kDefer>(ph, L.phase0, taken, G, P, rng, au, av, fr, hook) ? 1u : 0u;
         return true;
     }
     ++n.rejected;
@@ -1271,12 +1333,13 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
     return false;
 }
 
-template <int kIntegrator, int kSep, class Rhs = RhsExact, int kMixed = 0>
+template <int kIntegrator, int kSep, class Rhs = RhsExact, int kMixed = 0, int kDefer = 0>
 GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, PhotonRng& rng,
-                         LaneCounters& n, double& au, double& av)
+                         LaneCounters& n, double& au, double& av, double& fr)
 {
     NoHook h;
-    return lane_iterate<kIntegrator, kSep, Rhs, kMixed, NoHook>(L, G, P, rng, n, au, av, h);
+    return lane_iterate<kIntegrator, kSep, Rhs, kMixed, NoHook, kDefer>(L, G, P, rng, n, au, av,
+                                                                        fr, h);
 }
 
 }  // namespace grt

@@ -73,6 +73,24 @@ __device__ __forceinline__ int64_t obs_bin(double v, double lo, double hi, int64
 constexpr int kLdsBins = 4096;
 
 // build_observation_products, observation.hpp:31-84 (sums by atomics).
+// Draws the thermal frequencies whose draw the tracing kernel deferred: the
+// five draws before the photon's final draw counter, at T = -freq, then the
+// arrival redshift factor for a photon that reached the camera.
+__global__ __launch_bounds__(256) void resolve_deferred(PhotonPtrs ph, uint64_t n, uint64_t seed,
+                                                        uint32_t stream)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !ph.fpend[i]) return;
+    Photon p;
+    p.freq = ph.freq[i];
+    p.phase = ph.phase[i];
+    p.pend = 1;
+    const PhotonRng rng{ph.id[i], seed, stream, ph.rng_draw[i]};
+    finish_deferred(p, rng, p.phase == kArrived ? ph.fratio[i] : 1.0);
+    ph.freq[i] = p.freq;
+    ph.fpend[i] = 0;
+}
+
 __global__ __launch_bounds__(256) void image_products(PhotonPtrs ph, uint64_t n, int64_t nx,
                                                       int64_t ny, int64_t bins, double smin,
                                                       double smax, double lmin, double lmax,
@@ -197,6 +215,17 @@ hipError_t launch_build_live(const uint8_t* term, uint64_t n, uint32_t* list, ui
     const uint64_t blocks = (n + per_block - 1) / per_block;
     hipLaunchKernelGGL(build_live, dim3((unsigned)blocks), dim3(block), 0, stream, term, n, list,
                        count);
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve_deferred(const PhotonPtrs& ph, uint64_t n, uint64_t seed,
+                                   uint32_t stream_id, hipStream_t stream)
+{
+    if (n == 0) return hipSuccess;
+    const int block = 256;
+    const uint64_t blocks = (n + block - 1) / block;
+    hipLaunchKernelGGL(resolve_deferred, dim3((unsigned)blocks), dim3(block), 0, stream, ph, n,
+                       seed, stream_id);
     return hipGetLastError();
 }
 

@@ -77,7 +77,7 @@ __device__ __forceinline__ void stage_fill(WaveStage* S, const TraceLaunch& a,
     S->v[14][l] = gmem(p.dl)[i];
     S->id[l] = gmem(p.id)[i];
     S->draw[l] = gmem(p.rng_draw)[i];
-    S->phase[l] = gmem(p.phase)[i];
+    S->phase[l] = (uint32_t)gmem(p.phase)[i] | ((uint32_t)gmem(p.fpend)[i] << 8);
     S->idx[l] = i;
 }
 
@@ -93,6 +93,7 @@ __device__ __forceinline__ uint32_t stage_read(const WaveStage* S, uint32_t pos,
     ph.eem = S->v[13][pos];
     ph.dl = S->v[14][pos];
     ph.phase = (uint8_t)S->phase[pos];
+    ph.pend = (uint8_t)(S->phase[pos] >> 8);
     ph.term = 0;
     rng.id = S->id[pos];
     rng.draw = S->draw[pos];
@@ -143,7 +144,7 @@ __device__ __forceinline__ bool stage_take(WaveStage* S, StageState& st, const T
 }
 
 __device__ __forceinline__ void store_photon(const PhotonPtrs& p, uint32_t idx, const Photon& ph,
-                                             const PhotonRng& rng, double au, double av)
+                                             const PhotonRng& rng, double au, double av, double fr)
 {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -159,8 +160,10 @@ __device__ __forceinline__ void store_photon(const PhotonPtrs& p, uint32_t idx, 
     if (ph.phase == kArrived) {
         gmem(p.udisp)[idx] = au;
         gmem(p.vdisp)[idx] = av;
+        if (ph.pend) gmem(p.fratio)[idx] = fr;
     }
     gmem(p.rng_draw)[idx] = rng.draw;
+    gmem(p.fpend)[idx] = ph.pend;
 }
 
 #ifndef GRT_TRACE_WAVES_PER_EU
@@ -296,7 +299,7 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a)
         if (kSep) G.tab = lds;
         bool need = false;
         if (have) {
-            double au = 0.0, av = 0.0;
+            double au = 0.0, av = 0.0, fr = 1.0;
             if (kTape && !L.in_call && ((uint32_t)thdr % kSegmentCalls) == 0) {
                 const uint32_t k = (uint32_t)(thdr >> 32) & 0x7FFFFFFFu;
                 if (k < (uint32_t)a.tape_slots)
@@ -307,9 +310,10 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a)
             bool call_done;
             if (kTape)
                 call_done = lane_iterate<kIntegrator, kSep, Rhs, kMixed, TapeFlag>(
-                    L, G, P, rng, n, au, av, flag);
+                    L, G, P, rng, n, au, av, fr, flag);
             else
-                call_done = lane_iterate<kIntegrator, kSep, Rhs, kMixed>(L, G, P, rng, n, au, av);
+                call_done = lane_iterate<kIntegrator, kSep, Rhs, kMixed, 1>(L, G, P, rng, n, au,
+                                                                            av, fr);
             if (kTape && flag.touched) thdr |= kTapeActive;
             if (call_done) {
                 ++n.calls;
@@ -319,7 +323,7 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a)
                     thdr = tape_end_call(thdr, false, L.ph.term != 0, a.tape_slots,
                                          a.tape_overflow);
                 if (L.ph.term || calls_left == 0) {
-                    store_photon(PH, idx, L.ph, rng, au, av);
+                    store_photon(PH, idx, L.ph, rng, au, av, fr);
                     if (kTape) a.tape_hdr[idx] = thdr;
                     if (L.ph.term) {
                         const uint32_t step = (uint32_t)(a.step0 + calls_done - 1);
