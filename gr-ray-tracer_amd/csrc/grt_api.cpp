@@ -608,13 +608,8 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
     }
     if (n > 0) {
         // Thermal frequencies whose draw the tracing kernel deferred.
-        GRT_HIP(c, hipEventRecord(c->ev0, stream));
         GRT_HIP(c, grt::launch_resolve_deferred(c->h_consts.ph, n, P.seed, P.stream << 1, stream));
-        GRT_HIP(c, hipEventRecord(c->ev1, stream));
         GRT_HIP(c, hipStreamSynchronize(stream));
-        float ms = 0.0f;
-        GRT_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        kernel_ms += ms;
     }
     if (tape_slots > 0) {
         GRT_HIP(c, grt::launch_tape_finalize(c->d_thdr, n, tape_slots, c->d_counters + 3, stream));

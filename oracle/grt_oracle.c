@@ -322,6 +322,7 @@ int orc_rk45_generic(int n, double *state, double *dt, const double *k1, orc_rhs
     const double h = *dt;
     double k2[ORC_NMAX], k3[ORC_NMAX], k4[ORC_NMAX], k5[ORC_NMAX], k6[ORC_NMAX], tmp[ORC_NMAX];
     double x4[ORC_NMAX], x5[ORC_NMAX];
+    if (n < 1 || n > ORC_NMAX) return 0;
     for (int i = 0; i < n; ++i) tmp[i] = state[i] + h * A21 * k1[i];
     f(tmp, k2, user);
     for (int i = 0; i < n; ++i) tmp[i] = state[i] + h * (A31 * k1[i] + A32 * k2[i]);

@@ -145,6 +145,9 @@ def test_work_list_edges(oracle, gpu, c1):
             check_states(g, o, frac=1.0)
             for k in ("x0", "x1", "k1", "I", "dlambda"):
                 assert np.array_equal(g[k], o[k]), (n, tune, k)
+            # thermal frequencies drawn at emission (log() may differ by an ulp)
+            assert np.allclose(g["frequency"], o["frequency"], rtol=1e-14, atol=0.0,
+                               equal_nan=True), (n, tune)
             for k in ("functor_calls", "accepted_steps", "rejected_attempts", "global_steps",
                       "terminated"):
                 assert st_g[k] == st_o[k], (n, tune, k)
@@ -203,8 +206,34 @@ def test_termination_fraction_schedule(oracle, gpu, c1):
     for thr, interval, max_steps in ((0.5, 5, 10000), (0.9, 1, 10000), (1.0, 3, 17), (0.2, 7, 9)):
         g, o, st_g, st_o = trace_both(oracle, ctx, po, G, ph0, max_steps, thr, interval)
         check_states(g, o, frac=1.0)
+        # photons stopped between emission and the camera keep their
+        # emitted frequency; arrived ones carry the redshift factor
+        assert np.allclose(g["frequency"], o["frequency"], rtol=1e-14, atol=0.0,
+                           equal_nan=True), (thr, interval)
         assert st_g["global_steps"] == st_o["global_steps"], (thr, interval)
         assert st_g["accepted_steps"] == st_o["accepted_steps"], (thr, interval)
+
+
+def test_trace_resumes_after_stop(oracle, gpu, c1):
+    """A trace stopped by max_steps between emission and the camera (the
+    photon's frequency is drawn before the call returns) and then traced
+    again to the end ends exactly as in the oracle."""
+    import grt_amd
+    kw = c1_params()
+    su, sv = pixel_screen(24)
+    ctx = gpu_ctx(kw, c1)
+    po, G = oracle_parts(oracle, kw, c1)
+    ph0 = oracle.init_image(po, su, sv, 0.1)
+    g, o, _, _ = trace_both(oracle, ctx, po, G, ph0, max_steps=17)
+    assert np.count_nonzero(o["phase"] == 2) > 100      # emitted, not yet at the camera
+    assert np.allclose(g["frequency"], o["frequency"], rtol=1e-14, atol=0.0, equal_nan=True)
+    gd = grt_amd.to_device(g)
+    ctx.trace(gd, 10000)
+    g = grt_amd.to_host(gd)
+    oracle.trace(po, G, o, 10000, 1.0, 0, 8)
+    check_states(g, o, frac=1.0)
+    assert np.allclose(g["frequency"], o["frequency"], rtol=1e-14, atol=0.0, equal_nan=True)
+    check_image(ctx, oracle, g, o, 24)
 
 
 def test_scattering_image(oracle, gpu, c1):
