@@ -154,6 +154,14 @@ GRT_HD double rcp_rn(double d)
 // Kerr-Schild form, kerr_schild_core.hpp:161-192 (with r, dr/dx^k of :36-46,
 // l, dl/dx^k of :59-77 and H, dH/dx^k of :86-95 expanded for k = x, y, z).
 // out[4] (dp_t/dlambda) is identically zero and is not written.
+//
+// Rounding: every product that the reference's expressions add to a sum is
+// fused with that addition (an explicit fma: one rounding, as a contracting
+// compiler builds the reference), every quotient is the correctly rounded
+// IEEE quotient and every root the correctly rounded root.  The oracle's
+// integrator evaluates the same operations in the same order, so the two
+// agree bit for bit; the file is compiled without automatic contraction so
+// that no other operation is fused.
 
 GRT_HD void ks_rhs(const double s[8], double a, double m, double out[8])
 {
@@ -161,17 +169,18 @@ GRT_HD void ks_rhs(const double s[8], double a, double m, double out[8])
     const double p0 = s[4], p1 = s[5], p2 = s[6], p3 = s[7];
     const double a2 = a * a;
 
-    const double A = (x * x + y * y + z * z) - a2;
+    const double A = fma(z, z, fma(y, y, x * x)) - a2;
     const double az = a * z;
+    const double az2 = az * az;
     // S > 0 and r > 0 away from the ring singularity, which photons never
     // reach (they terminate at r_term_min)
-    const double S = sqrt_rn(A * A + 4.0 * (az * az));
+    const double S = sqrt_rn(fma(A, A, 4.0 * az2));
     const double r = sqrt_rn(0.5 * (A + S));
     const double tr = 2.0 * r;
-    const double B = r * r + a2;
+    const double B = fma(r, r, a2);
     const double r2 = r * r;
     const double r3 = r2 * r;
-    const double D = r3 * r + az * az;
+    const double D = fma(r3, r, az2);
     // Every quotient n/d is the correctly rounded IEEE quotient, evaluated
     // from the correctly rounded reciprocal y = 1/d of its divisor with
     // Markstein's correction (div_rn), so the five divisors cost one
@@ -180,47 +189,48 @@ GRT_HD void ks_rhs(const double s[8], double a, double m, double out[8])
     const double iS = rcp_rn(S), ir = rcp_rn(r), iB = rcp_rn(B), iD = rcp_rn(D), iD2 = rcp_rn(D2);
     const double itr = 0.5 * ir;
     const double f = 1.0 + div_rn(A, S, iS);
+    const double a2z2 = 2.0 * a2 * z;
     const double drx = div_rn(x * f, tr, itr);
     const double dry = div_rn(y * f, tr, itr);
-    const double drz = div_rn(z * f + div_rn(2.0 * a2 * z, S, iS), tr, itr);
+    const double drz = div_rn(fma(z, f, div_rn(a2z2, S, iS)), tr, itr);
 
-    const double l1 = div_rn(r * x + a * y, B, iB);
-    const double l2 = div_rn(r * y - a * x, B, iB);
+    const double l1 = div_rn(fma(r, x, a * y), B, iB);
+    const double l2 = div_rn(fma(r, y, -(a * x)), B, iB);
     const double l3 = div_rn(z, r, ir);
     const double dBx = tr * drx, dBy = tr * dry, dBz = tr * drz;
     // dl[k][mu]; the Kronecker terms of :73-75 reduce to +-r, +-a or 1.
-    const double dl_x1 = div_rn((drx * x + r) - l1 * dBx, B, iB);
-    const double dl_x2 = div_rn((drx * y - a) - l2 * dBx, B, iB);
+    const double dl_x1 = div_rn(fma(-l1, dBx, fma(drx, x, r)), B, iB);
+    const double dl_x2 = div_rn(fma(-l2, dBx, fma(drx, y, -a)), B, iB);
     const double dl_x3 = div_rn(-(l3 * drx), r, ir);
-    const double dl_y1 = div_rn((dry * x + a) - l1 * dBy, B, iB);
-    const double dl_y2 = div_rn((dry * y + r) - l2 * dBy, B, iB);
+    const double dl_y1 = div_rn(fma(-l1, dBy, fma(dry, x, a)), B, iB);
+    const double dl_y2 = div_rn(fma(-l2, dBy, fma(dry, y, r)), B, iB);
     const double dl_y3 = div_rn(-(l3 * dry), r, ir);
-    const double dl_z1 = div_rn(drz * x - l1 * dBz, B, iB);
-    const double dl_z2 = div_rn(drz * y - l2 * dBz, B, iB);
-    const double dl_z3 = div_rn(1.0 - l3 * drz, r, ir);
+    const double dl_z1 = div_rn(fma(-l1, dBz, drz * x), B, iB);
+    const double dl_z2 = div_rn(fma(-l2, dBz, drz * y), B, iB);
+    const double dl_z3 = div_rn(fma(-l3, drz, 1.0), r, ir);
 
     const double H = div_rn(m * r3, D, iD);
     const double c3 = 3.0 * r2;
     const double c4 = 4.0 * r3;
-    const double dH_x = div_rn(m * (c3 * drx * D - r3 * (c4 * drx)), D2, iD2);
-    const double dH_y = div_rn(m * (c3 * dry * D - r3 * (c4 * dry)), D2, iD2);
-    const double dH_z = div_rn(m * (c3 * drz * D - r3 * (c4 * drz + 2.0 * a2 * z)), D2, iD2);
+    const double dH_x = div_rn(m * fma(c3 * drx, D, -(r3 * (c4 * drx))), D2, iD2);
+    const double dH_y = div_rn(m * fma(c3 * dry, D, -(r3 * (c4 * dry))), D2, iD2);
+    const double dH_z = div_rn(m * fma(c3 * drz, D, -(r3 * fma(c4, drz, a2z2))), D2, iD2);
 
     // l^mu = (-1, l1, l2, l3)
-    const double lp = ((-p0 + l1 * p1) + l2 * p2) + l3 * p3;
+    const double lp = fma(l3, p3, fma(l2, p2, fma(l1, p1, -p0)));
     const double twoH = 2.0 * H;
-    out[0] = -p0 + twoH * lp;
-    out[1] = p1 - twoH * l1 * lp;
-    out[2] = p2 - twoH * l2 * lp;
-    out[3] = p3 - twoH * l3 * lp;
-    const double dlp_x = (dl_x1 * p1 + dl_x2 * p2) + dl_x3 * p3;
-    const double dlp_y = (dl_y1 * p1 + dl_y2 * p2) + dl_y3 * p3;
-    const double dlp_z = (dl_z1 * p1 + dl_z2 * p2) + dl_z3 * p3;
+    out[0] = fma(twoH, lp, -p0);
+    out[1] = fma(-(twoH * l1), lp, p1);
+    out[2] = fma(-(twoH * l2), lp, p2);
+    out[3] = fma(-(twoH * l3), lp, p3);
+    const double dlp_x = fma(dl_x3, p3, fma(dl_x2, p2, dl_x1 * p1));
+    const double dlp_y = fma(dl_y3, p3, fma(dl_y2, p2, dl_y1 * p1));
+    const double dlp_z = fma(dl_z3, p3, fma(dl_z2, p2, dl_z1 * p1));
     const double lp2 = lp * lp;
     const double hl = twoH * lp;
-    out[5] = dH_x * lp2 + hl * dlp_x;
-    out[6] = dH_y * lp2 + hl * dlp_y;
-    out[7] = dH_z * lp2 + hl * dlp_z;
+    out[5] = fma(hl, dlp_x, dH_x * lp2);
+    out[6] = fma(hl, dlp_y, dH_y * lp2);
+    out[7] = fma(hl, dlp_z, dH_z * lp2);
 }
 
 // Hamilton's equations of ks_rhs in fewer operations (mixed-precision form).
@@ -338,16 +348,17 @@ template <class Rhs = RhsExact>
 GRT_HD void rk4_step(double s[8], double h, const StepParams& P)
 {
     double k1[8], k2[8], k3[8], k4[8], t[8];
+    const double h2 = 0.5 * h;
     Rhs::eval(s, P, k1);
     t[4] = s[4];
-    GRT_FOR_STATE(i) t[i] = s[i] + 0.5 * h * k1[i];
+    GRT_FOR_STATE(i) t[i] = fma(h2, k1[i], s[i]);
     Rhs::eval(t, P, k2);
-    GRT_FOR_STATE(i) t[i] = s[i] + 0.5 * h * k2[i];
+    GRT_FOR_STATE(i) t[i] = fma(h2, k2[i], s[i]);
     Rhs::eval(t, P, k3);
-    GRT_FOR_STATE(i) t[i] = s[i] + h * k3[i];
+    GRT_FOR_STATE(i) t[i] = fma(h, k3[i], s[i]);
     Rhs::eval(t, P, k4);
     const double h6 = h / 6.0;
-    GRT_FOR_STATE(i) s[i] += h6 * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]);
+    GRT_FOR_STATE(i) s[i] = fma(h6, k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i], s[i]);
 }
 
 // One RKF45 attempt with precomputed k1, integrate_cart_ks.hpp:160-227.
@@ -357,7 +368,8 @@ GRT_HD void rk4_step(double s[8], double h, const StepParams& P)
 // The stage sums are accumulated left to right as each k_j becomes
 // available (P4 = a41 k1 + a42 k2, later P4 + a43 k3, ...), which is exactly
 // the evaluation order of the reference's expressions, so only the partial
-// sums -- not k2..k6 -- stay live in registers.
+// sums -- not k2..k6 -- stay live in registers.  Each product joins its sum
+// in one fma (the rounding of ks_rhs), as does the final s + h * sum.
 //
 // kMixed evaluates the controller's err^(-1/4) with the hardware's
 // approximate FP64 square root and reciprocal instead of the correctly
@@ -369,36 +381,37 @@ template <class Rhs = RhsExact, int kMixed = 0>
 GRT_HD bool rk45_attempt(double s[8], double* h_io, const double k1[8], const StepParams& P)
 {
     const double h = *h_io;
+    const double h21 = h * A21;
     double k[8], t[8], P4[8], P5[8], P6[8], X4[8], X5[8];
     t[4] = s[4];
-    GRT_FOR_STATE(i) t[i] = s[i] + h * A21 * k1[i];
+    GRT_FOR_STATE(i) t[i] = fma(h21, k1[i], s[i]);
     Rhs::eval(t, P, k);                                   // k2
     GRT_FOR_STATE(i) {
-        t[i] = s[i] + h * (A31 * k1[i] + A32 * k[i]);
-        P4[i] = A41 * k1[i] + A42 * k[i];
-        P5[i] = A51 * k1[i] + A52 * k[i];
-        P6[i] = A61 * k1[i] + A62 * k[i];
+        t[i] = fma(h, fma(A32, k[i], A31 * k1[i]), s[i]);
+        P4[i] = fma(A42, k[i], A41 * k1[i]);
+        P5[i] = fma(A52, k[i], A51 * k1[i]);
+        P6[i] = fma(A62, k[i], A61 * k1[i]);
     }
     Rhs::eval(t, P, k);                                   // k3
     GRT_FOR_STATE(i) {
-        t[i] = s[i] + h * (P4[i] + A43 * k[i]);
-        P5[i] = P5[i] + A53 * k[i];
-        P6[i] = P6[i] + A63 * k[i];
-        X4[i] = B41 * k1[i] + B43 * k[i];
-        X5[i] = B51 * k1[i] + B53 * k[i];
+        t[i] = fma(h, fma(A43, k[i], P4[i]), s[i]);
+        P5[i] = fma(A53, k[i], P5[i]);
+        P6[i] = fma(A63, k[i], P6[i]);
+        X4[i] = fma(B43, k[i], B41 * k1[i]);
+        X5[i] = fma(B53, k[i], B51 * k1[i]);
     }
     Rhs::eval(t, P, k);                                   // k4
     GRT_FOR_STATE(i) {
-        t[i] = s[i] + h * (P5[i] + A54 * k[i]);
-        P6[i] = P6[i] + A64 * k[i];
-        X4[i] = X4[i] + B44 * k[i];
-        X5[i] = X5[i] + B54 * k[i];
+        t[i] = fma(h, fma(A54, k[i], P5[i]), s[i]);
+        P6[i] = fma(A64, k[i], P6[i]);
+        X4[i] = fma(B44, k[i], X4[i]);
+        X5[i] = fma(B54, k[i], X5[i]);
     }
     Rhs::eval(t, P, k);                                   // k5
     GRT_FOR_STATE(i) {
-        t[i] = s[i] + h * (P6[i] + A65 * k[i]);
-        X4[i] = X4[i] + B45 * k[i];
-        X5[i] = X5[i] + B55 * k[i];
+        t[i] = fma(h, fma(A65, k[i], P6[i]), s[i]);
+        X4[i] = fma(B45, k[i], X4[i]);
+        X5[i] = fma(B55, k[i], X5[i]);
     }
     Rhs::eval(t, P, k);                                   // k6
 
@@ -407,9 +420,9 @@ GRT_HD bool rk45_attempt(double s[8], double* h_io, const double k1[8], const St
     bool finite = is_finite(s[4]);
     x5[4] = s[4];
     GRT_FOR_STATE(i) {
-        const double x4 = s[i] + h * X4[i];
-        x5[i] = s[i] + h * (X5[i] + B56 * k[i]);
-        const double sc = P.atol + P.rtol * fmax(fabs(x5[i]), fabs(x4));
+        const double x4 = fma(h, X4[i], s[i]);
+        x5[i] = fma(h, fma(B56, k[i], X5[i]), s[i]);
+        const double sc = fma(P.rtol, fmax(fabs(x5[i]), fabs(x4)), P.atol);
         const double ratio = fabs(x5[i] - x4) / sc;
         finite = finite && is_finite(x5[i]) && is_finite(ratio);
         err = fmax(err, ratio);

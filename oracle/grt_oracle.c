@@ -3,14 +3,19 @@
  *
  * TEST INFRASTRUCTURE ONLY: loaded by tests/, by the smoke check and by the
  * CPU-baseline leg of the benchmark, always as the checker, never as the thing
- * measured on the GPU.  Built with gcc -O2 -fno-fast-math and no FMA so every
- * expression is evaluated in the order written, which is the reference's
- * order.  Citations are file:line inside the reference's src/ directory.
+ * measured on the GPU.  Built with gcc -O2 -fno-fast-math and no automatic
+ * FMA contraction, so every expression is evaluated in the order written,
+ * which is the reference's order; the only fused operations are the explicit
+ * fma() calls of the integrators' right-hand side and stage sums (one
+ * rounding per product-plus-sum, as a contracting compiler builds the
+ * reference).  Citations are file:line inside the reference's src/ directory.
  *
  * Parity status of each function (how it is pinned, see tests/test_oracle_*.py):
  *   metric core / RHS ............ closed forms at 50 digits (mpmath), g*ginv = I,
  *                                  null l, RHS == ginv p and -1/2 dginv pp
  *                                  (test_geodesic_observables.cpp:47-66)
+ *   fused RHS .................... within a few ulp of the RHS above, and the
+ *                                  same closed forms
  *   RK4 / RK45 ................... exact linear system (test_rk_integrators.cpp),
  *                                  convergence order + tolerance monotonicity
  *                                  (test_convergence.cpp), hand accept/reject cases
@@ -191,6 +196,75 @@ void orc_hamiltonian_rhs(const double X[4], const double p[4], double a, double 
     }
 }
 
+/* The same right-hand side as orc_hamiltonian_rhs (kerr_schild_core.hpp:
+ * 161-192 with :36-46, :59-77 and :86-95 written out for k = x, y, z), with
+ * the rounding the integrators use: every product that the reference's
+ * expressions add to a sum is fused with that addition (fma, one rounding,
+ * as a contracting compiler builds the reference); quotients and roots are
+ * the IEEE operations.  Zero Kronecker terms are dropped (adding or
+ * multiplying by them is exact).  A test holds it within a few ulp of
+ * orc_hamiltonian_rhs. */
+void orc_hamiltonian_rhs_fused(const double s[8], double a, double m, double out[8])
+{
+    const double x = s[1], y = s[2], z = s[3];
+    const double p0 = s[4], p1 = s[5], p2 = s[6], p3 = s[7];
+    const double a2 = a * a;
+    /* compute_r_and_deriv */
+    const double A = fma(z, z, fma(y, y, x * x)) - a2;
+    const double az = a * z;
+    const double az2 = az * az;
+    const double S = sqrt(fma(A, A, 4.0 * az2));
+    const double r = sqrt(0.5 * (A + S));
+    const double tr = 2.0 * r;
+    const double f = 1.0 + A / S;
+    const double a2z2 = 2.0 * a2 * z;
+    const double drx = x * f / tr;
+    const double dry = y * f / tr;
+    const double drz = fma(z, f, a2z2 / S) / tr;
+    /* compute_l_and_deriv */
+    const double B = fma(r, r, a2);
+    const double l1 = fma(r, x, a * y) / B;
+    const double l2 = fma(r, y, -(a * x)) / B;
+    const double l3 = z / r;
+    const double dBx = tr * drx, dBy = tr * dry, dBz = tr * drz;
+    const double dl_x1 = fma(-l1, dBx, fma(drx, x, r)) / B;
+    const double dl_x2 = fma(-l2, dBx, fma(drx, y, -a)) / B;
+    const double dl_x3 = -(l3 * drx) / r;
+    const double dl_y1 = fma(-l1, dBy, fma(dry, x, a)) / B;
+    const double dl_y2 = fma(-l2, dBy, fma(dry, y, r)) / B;
+    const double dl_y3 = -(l3 * dry) / r;
+    const double dl_z1 = fma(-l1, dBz, drz * x) / B;
+    const double dl_z2 = fma(-l2, dBz, drz * y) / B;
+    const double dl_z3 = fma(-l3, drz, 1.0) / r;
+    /* compute_H_and_deriv */
+    const double r2 = r * r;
+    const double r3 = r2 * r;
+    const double D = fma(r3, r, az2);
+    const double D2 = D * D;
+    const double H = m * r3 / D;
+    const double c3 = 3.0 * r2;
+    const double c4 = 4.0 * r3;
+    const double dH_x = m * fma(c3 * drx, D, -(r3 * (c4 * drx))) / D2;
+    const double dH_y = m * fma(c3 * dry, D, -(r3 * (c4 * dry))) / D2;
+    const double dH_z = m * fma(c3 * drz, D, -(r3 * fma(c4, drz, a2z2))) / D2;
+    /* compute_hamiltonian_rhs; l^mu = (-1, l1, l2, l3) */
+    const double lp = fma(l3, p3, fma(l2, p2, fma(l1, p1, -p0)));
+    const double twoH = 2.0 * H;
+    out[0] = fma(twoH, lp, -p0);
+    out[1] = fma(-(twoH * l1), lp, p1);
+    out[2] = fma(-(twoH * l2), lp, p2);
+    out[3] = fma(-(twoH * l3), lp, p3);
+    out[4] = 0.0;
+    const double dlp_x = fma(dl_x3, p3, fma(dl_x2, p2, dl_x1 * p1));
+    const double dlp_y = fma(dl_y3, p3, fma(dl_y2, p2, dl_y1 * p1));
+    const double dlp_z = fma(dl_z3, p3, fma(dl_z2, p2, dl_z1 * p1));
+    const double lp2 = lp * lp;
+    const double hl = twoH * lp;
+    out[5] = fma(hl, dlp_x, dH_x * lp2);
+    out[6] = fma(hl, dlp_y, dH_y * lp2);
+    out[7] = fma(hl, dlp_z, dH_z * lp2);
+}
+
 /* kerr_schild_core.hpp:201-227 */
 int orc_null_covariant_momentum(const double X[4], const double K[3], double a, double m, double p[4])
 {
@@ -305,14 +379,16 @@ static double quarter_root(double x)
 void orc_rk4_generic(int n, double *state, double dt, orc_rhs_fn f, void *user)
 {
     double k1[ORC_NMAX], k2[ORC_NMAX], k3[ORC_NMAX], k4[ORC_NMAX], tmp[ORC_NMAX];
+    if (n < 1 || n > ORC_NMAX) return;
     f(state, k1, user);
-    for (int i = 0; i < n; ++i) tmp[i] = state[i] + 0.5 * dt * k1[i];
+    for (int i = 0; i < n; ++i) tmp[i] = fma(0.5 * dt, k1[i], state[i]);
     f(tmp, k2, user);
-    for (int i = 0; i < n; ++i) tmp[i] = state[i] + 0.5 * dt * k2[i];
+    for (int i = 0; i < n; ++i) tmp[i] = fma(0.5 * dt, k2[i], state[i]);
     f(tmp, k3, user);
-    for (int i = 0; i < n; ++i) tmp[i] = state[i] + dt * k3[i];
+    for (int i = 0; i < n; ++i) tmp[i] = fma(dt, k3[i], state[i]);
     f(tmp, k4, user);
-    for (int i = 0; i < n; ++i) state[i] += (dt / 6.0) * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]);
+    for (int i = 0; i < n; ++i)
+        state[i] = fma(dt / 6.0, k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i], state[i]);
 }
 
 /* rk_integrators.hpp:35-106 (== integrate_cart_ks.hpp:160-227); returns accepted */
@@ -323,24 +399,31 @@ int orc_rk45_generic(int n, double *state, double *dt, const double *k1, orc_rhs
     double k2[ORC_NMAX], k3[ORC_NMAX], k4[ORC_NMAX], k5[ORC_NMAX], k6[ORC_NMAX], tmp[ORC_NMAX];
     double x4[ORC_NMAX], x5[ORC_NMAX];
     if (n < 1 || n > ORC_NMAX) return 0;
-    for (int i = 0; i < n; ++i) tmp[i] = state[i] + h * A21 * k1[i];
+    /* each product joins its running sum in one fma (left to right, as
+     * written), and so does the final state + h * sum */
+    for (int i = 0; i < n; ++i) tmp[i] = fma(h * A21, k1[i], state[i]);
     f(tmp, k2, user);
-    for (int i = 0; i < n; ++i) tmp[i] = state[i] + h * (A31 * k1[i] + A32 * k2[i]);
+    for (int i = 0; i < n; ++i) tmp[i] = fma(h, fma(A32, k2[i], A31 * k1[i]), state[i]);
     f(tmp, k3, user);
-    for (int i = 0; i < n; ++i) tmp[i] = state[i] + h * (A41 * k1[i] + A42 * k2[i] + A43 * k3[i]);
+    for (int i = 0; i < n; ++i)
+        tmp[i] = fma(h, fma(A43, k3[i], fma(A42, k2[i], A41 * k1[i])), state[i]);
     f(tmp, k4, user);
     for (int i = 0; i < n; ++i)
-        tmp[i] = state[i] + h * (A51 * k1[i] + A52 * k2[i] + A53 * k3[i] + A54 * k4[i]);
+        tmp[i] = fma(h, fma(A54, k4[i], fma(A53, k3[i], fma(A52, k2[i], A51 * k1[i]))), state[i]);
     f(tmp, k5, user);
     for (int i = 0; i < n; ++i)
-        tmp[i] = state[i] + h * (A61 * k1[i] + A62 * k2[i] + A63 * k3[i] + A64 * k4[i] + A65 * k5[i]);
+        tmp[i] = fma(h, fma(A65, k5[i], fma(A64, k4[i], fma(A63, k3[i],
+                                                             fma(A62, k2[i], A61 * k1[i])))),
+                     state[i]);
     f(tmp, k6, user);
     double err = 0.0;
     int finite = 1;
     for (int i = 0; i < n; ++i) {
-        x4[i] = state[i] + h * (B41 * k1[i] + B43 * k3[i] + B44 * k4[i] + B45 * k5[i]);
-        x5[i] = state[i] + h * (B51 * k1[i] + B53 * k3[i] + B54 * k4[i] + B55 * k5[i] + B56 * k6[i]);
-        const double sc = atol + rtol * fmax(fabs(x5[i]), fabs(x4[i]));
+        x4[i] = fma(h, fma(B45, k5[i], fma(B44, k4[i], fma(B43, k3[i], B41 * k1[i]))), state[i]);
+        x5[i] = fma(h, fma(B56, k6[i], fma(B55, k5[i], fma(B54, k4[i], fma(B53, k3[i],
+                                                                             B51 * k1[i])))),
+                    state[i]);
+        const double sc = fma(rtol, fmax(fabs(x5[i]), fabs(x4[i])), atol);
         const double ratio = fabs(x5[i] - x4[i]) / sc;
         if (!isfinite(x5[i]) || !isfinite(ratio)) finite = 0;
         err = fmax(err, ratio);
@@ -369,7 +452,7 @@ typedef struct { double a, m; } ks_user;
 static void ks_rhs_cb(const double *in, double *out, void *user)
 {
     const ks_user *u = (const ks_user *)user;
-    orc_hamiltonian_rhs(in, in + 4, u->a, u->m, out);
+    orc_hamiltonian_rhs_fused(in, u->a, u->m, out);
 }
 
 /* integrate_cart_ks.hpp:143-155 */
@@ -903,7 +986,7 @@ static void step_photon(const orc_params *p, const orc_grid *g, orc_photons *ph,
         st->accepted_steps++;
     } else if (p->integrator == 1) {
         double k1[8];
-        orc_hamiltonian_rhs(s, s + 4, p->a_bh, p->m_bh, k1);
+        orc_hamiltonian_rhs_fused(s, p->a_bh, p->m_bh, k1);
         double dl = ph->dlambda[idx];
         int acc = 0;
         for (int attempt = 0; attempt < 12; ++attempt) {

@@ -112,6 +112,7 @@ def lib():
             "orc_compute_inverse_metric": (None, [_D, C.c_double, C.c_double, _D]),
             "orc_compute_inverse_metric_deriv": (None, [_D, C.c_double, C.c_double, _D]),
             "orc_hamiltonian_rhs": (None, [_D, _D, C.c_double, C.c_double, _D]),
+            "orc_hamiltonian_rhs_fused": (None, [_D, C.c_double, C.c_double, _D]),
             "orc_null_covariant_momentum": (C.c_int, [_D, _D, C.c_double, C.c_double, _D]),
             "orc_inverse_norm": (C.c_double, [_D, _D, C.c_double, C.c_double]),
             "orc_geodesic_observables": (None, [_D, C.c_double, C.c_double, _D]),
@@ -322,6 +323,14 @@ def rhs(X, p, a, m):
     p = np.ascontiguousarray(p, dtype=np.float64)
     out = np.zeros(8)
     lib().orc_hamiltonian_rhs(dptr(X), dptr(p), a, m, dptr(out))
+    return out
+
+
+def rhs_fused(X, p, a, m):
+    """The right-hand side as the integrators evaluate it (fused rounding)."""
+    s = np.ascontiguousarray(np.concatenate([X, p]), dtype=np.float64)
+    out = np.zeros(8)
+    lib().orc_hamiltonian_rhs_fused(dptr(s), a, m, dptr(out))
     return out
 
 
