@@ -375,11 +375,32 @@ static double quarter_root(double x)
 }
 
 
+/* Rounding model of the integrators.  0 (default): the fused model that
+ * the tracing kernel reproduces bit for bit (explicit fma in the RHS and the
+ * stage sums).  1: the reference's expressions rounded once per operation,
+ * unfused.  Both are builds of the reference (a contracting compiler and a
+ * non-contracting one); tests compare whole renders under the two. */
+static int g_unfused = 0;
+
+void orc_set_unfused_rounding(int unfused) { g_unfused = unfused ? 1 : 0; }
+
 /* rk_integrators.hpp:12-22 */
 void orc_rk4_generic(int n, double *state, double dt, orc_rhs_fn f, void *user)
 {
     double k1[ORC_NMAX], k2[ORC_NMAX], k3[ORC_NMAX], k4[ORC_NMAX], tmp[ORC_NMAX];
     if (n < 1 || n > ORC_NMAX) return;
+    if (g_unfused) {
+        f(state, k1, user);
+        for (int i = 0; i < n; ++i) tmp[i] = state[i] + 0.5 * dt * k1[i];
+        f(tmp, k2, user);
+        for (int i = 0; i < n; ++i) tmp[i] = state[i] + 0.5 * dt * k2[i];
+        f(tmp, k3, user);
+        for (int i = 0; i < n; ++i) tmp[i] = state[i] + dt * k3[i];
+        f(tmp, k4, user);
+        for (int i = 0; i < n; ++i)
+            state[i] += (dt / 6.0) * (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]);
+        return;
+    }
     f(state, k1, user);
     for (int i = 0; i < n; ++i) tmp[i] = fma(0.5 * dt, k1[i], state[i]);
     f(tmp, k2, user);
@@ -399,31 +420,59 @@ int orc_rk45_generic(int n, double *state, double *dt, const double *k1, orc_rhs
     double k2[ORC_NMAX], k3[ORC_NMAX], k4[ORC_NMAX], k5[ORC_NMAX], k6[ORC_NMAX], tmp[ORC_NMAX];
     double x4[ORC_NMAX], x5[ORC_NMAX];
     if (n < 1 || n > ORC_NMAX) return 0;
+    if (g_unfused) {
+        for (int i = 0; i < n; ++i) tmp[i] = state[i] + h * A21 * k1[i];
+        f(tmp, k2, user);
+        for (int i = 0; i < n; ++i) tmp[i] = state[i] + h * (A31 * k1[i] + A32 * k2[i]);
+        f(tmp, k3, user);
+        for (int i = 0; i < n; ++i)
+            tmp[i] = state[i] + h * (A41 * k1[i] + A42 * k2[i] + A43 * k3[i]);
+        f(tmp, k4, user);
+        for (int i = 0; i < n; ++i)
+            tmp[i] = state[i] + h * (A51 * k1[i] + A52 * k2[i] + A53 * k3[i] + A54 * k4[i]);
+        f(tmp, k5, user);
+        for (int i = 0; i < n; ++i)
+            tmp[i] = state[i] +
+                     h * (A61 * k1[i] + A62 * k2[i] + A63 * k3[i] + A64 * k4[i] + A65 * k5[i]);
+        f(tmp, k6, user);
+        for (int i = 0; i < n; ++i) {
+            x4[i] = state[i] + h * (B41 * k1[i] + B43 * k3[i] + B44 * k4[i] + B45 * k5[i]);
+            x5[i] = state[i] +
+                    h * (B51 * k1[i] + B53 * k3[i] + B54 * k4[i] + B55 * k5[i] + B56 * k6[i]);
+        }
+    }
     /* each product joins its running sum in one fma (left to right, as
      * written), and so does the final state + h * sum */
-    for (int i = 0; i < n; ++i) tmp[i] = fma(h * A21, k1[i], state[i]);
-    f(tmp, k2, user);
-    for (int i = 0; i < n; ++i) tmp[i] = fma(h, fma(A32, k2[i], A31 * k1[i]), state[i]);
-    f(tmp, k3, user);
-    for (int i = 0; i < n; ++i)
-        tmp[i] = fma(h, fma(A43, k3[i], fma(A42, k2[i], A41 * k1[i])), state[i]);
-    f(tmp, k4, user);
-    for (int i = 0; i < n; ++i)
-        tmp[i] = fma(h, fma(A54, k4[i], fma(A53, k3[i], fma(A52, k2[i], A51 * k1[i]))), state[i]);
-    f(tmp, k5, user);
-    for (int i = 0; i < n; ++i)
-        tmp[i] = fma(h, fma(A65, k5[i], fma(A64, k4[i], fma(A63, k3[i],
-                                                             fma(A62, k2[i], A61 * k1[i])))),
-                     state[i]);
-    f(tmp, k6, user);
+    else {
+        for (int i = 0; i < n; ++i) tmp[i] = fma(h * A21, k1[i], state[i]);
+        f(tmp, k2, user);
+        for (int i = 0; i < n; ++i) tmp[i] = fma(h, fma(A32, k2[i], A31 * k1[i]), state[i]);
+        f(tmp, k3, user);
+        for (int i = 0; i < n; ++i)
+            tmp[i] = fma(h, fma(A43, k3[i], fma(A42, k2[i], A41 * k1[i])), state[i]);
+        f(tmp, k4, user);
+        for (int i = 0; i < n; ++i)
+            tmp[i] = fma(h, fma(A54, k4[i], fma(A53, k3[i], fma(A52, k2[i], A51 * k1[i]))),
+                         state[i]);
+        f(tmp, k5, user);
+        for (int i = 0; i < n; ++i)
+            tmp[i] = fma(h, fma(A65, k5[i], fma(A64, k4[i], fma(A63, k3[i],
+                                                                 fma(A62, k2[i], A61 * k1[i])))),
+                         state[i]);
+        f(tmp, k6, user);
+        for (int i = 0; i < n; ++i) {
+            x4[i] = fma(h, fma(B45, k5[i], fma(B44, k4[i], fma(B43, k3[i], B41 * k1[i]))),
+                        state[i]);
+            x5[i] = fma(h, fma(B56, k6[i], fma(B55, k5[i], fma(B54, k4[i], fma(B53, k3[i],
+                                                                                 B51 * k1[i])))),
+                        state[i]);
+        }
+    }
     double err = 0.0;
     int finite = 1;
     for (int i = 0; i < n; ++i) {
-        x4[i] = fma(h, fma(B45, k5[i], fma(B44, k4[i], fma(B43, k3[i], B41 * k1[i]))), state[i]);
-        x5[i] = fma(h, fma(B56, k6[i], fma(B55, k5[i], fma(B54, k4[i], fma(B53, k3[i],
-                                                                             B51 * k1[i])))),
-                    state[i]);
-        const double sc = fma(rtol, fmax(fabs(x5[i]), fabs(x4[i])), atol);
+        const double big = fmax(fabs(x5[i]), fabs(x4[i]));
+        const double sc = g_unfused ? atol + rtol * big : fma(rtol, big, atol);
         const double ratio = fabs(x5[i] - x4[i]) / sc;
         if (!isfinite(x5[i]) || !isfinite(ratio)) finite = 0;
         err = fmax(err, ratio);
@@ -452,7 +501,10 @@ typedef struct { double a, m; } ks_user;
 static void ks_rhs_cb(const double *in, double *out, void *user)
 {
     const ks_user *u = (const ks_user *)user;
-    orc_hamiltonian_rhs_fused(in, u->a, u->m, out);
+    if (g_unfused)
+        orc_hamiltonian_rhs(in, in + 4, u->a, u->m, out);
+    else
+        orc_hamiltonian_rhs_fused(in, u->a, u->m, out);
 }
 
 /* integrate_cart_ks.hpp:143-155 */
@@ -986,7 +1038,8 @@ static void step_photon(const orc_params *p, const orc_grid *g, orc_photons *ph,
         st->accepted_steps++;
     } else if (p->integrator == 1) {
         double k1[8];
-        orc_hamiltonian_rhs_fused(s, p->a_bh, p->m_bh, k1);
+        ks_user u = {p->a_bh, p->m_bh};
+        ks_rhs_cb(s, k1, &u);
         double dl = ph->dlambda[idx];
         int acc = 0;
         for (int attempt = 0; attempt < 12; ++attempt) {

@@ -100,6 +100,8 @@ void orc_compute_inverse_metric(const double X[4], double a, double m, double gi
 void orc_compute_inverse_metric_deriv(const double X[4], double a, double m, double dginv[64]);
 void orc_hamiltonian_rhs(const double X[4], const double p[4], double a, double m, double rhs[8]);
 void orc_hamiltonian_rhs_fused(const double state[8], double a, double m, double rhs[8]);
+/* 0 (default): the fused rounding the tracing kernel reproduces; 1: unfused */
+void orc_set_unfused_rounding(int unfused);
 int orc_null_covariant_momentum(const double X[4], const double K[3], double a, double m, double p[4]);
 double orc_inverse_norm(const double X[4], const double V[4], double a, double m);
 void orc_geodesic_observables(const double state[8], double a, double m, double out[4]);

@@ -113,6 +113,7 @@ def lib():
             "orc_compute_inverse_metric_deriv": (None, [_D, C.c_double, C.c_double, _D]),
             "orc_hamiltonian_rhs": (None, [_D, _D, C.c_double, C.c_double, _D]),
             "orc_hamiltonian_rhs_fused": (None, [_D, C.c_double, C.c_double, _D]),
+            "orc_set_unfused_rounding": (None, [C.c_int]),
             "orc_null_covariant_momentum": (C.c_int, [_D, _D, C.c_double, C.c_double, _D]),
             "orc_inverse_norm": (C.c_double, [_D, _D, C.c_double, C.c_double]),
             "orc_geodesic_observables": (None, [_D, C.c_double, C.c_double, _D]),
@@ -324,6 +325,12 @@ def rhs(X, p, a, m):
     out = np.zeros(8)
     lib().orc_hamiltonian_rhs(dptr(X), dptr(p), a, m, dptr(out))
     return out
+
+
+def set_unfused_rounding(unfused):
+    """Switch the integrators between the fused rounding (default, what the
+    tracing kernel reproduces) and the reference's expressions unfused."""
+    lib().orc_set_unfused_rounding(1 if unfused else 0)
 
 
 def rhs_fused(X, p, a, m):
