@@ -607,9 +607,9 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
         if (!exact_schedule && live == 0) break;
     }
     if (n > 0) {
-        // Thermal frequencies whose draw the tracing kernel deferred.
+        // Thermal frequencies whose draw the tracing kernel deferred (stream
+        // ordered; the statistics copy below waits for it).
         GRT_HIP(c, grt::launch_resolve_deferred(c->h_consts.ph, n, P.seed, P.stream << 1, stream));
-        GRT_HIP(c, hipStreamSynchronize(stream));
     }
     if (tape_slots > 0) {
         GRT_HIP(c, grt::launch_tape_finalize(c->d_thdr, n, tape_slots, c->d_counters + 3, stream));
@@ -625,7 +625,8 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
         c->tape_integrator = P.integrator;
     }
     unsigned long long hs[grt::kStatCount];
-    GRT_HIP(c, hipMemcpy(hs, c->d_stats, sizeof hs, hipMemcpyDeviceToHost));
+    GRT_HIP(c, hipMemcpyAsync(hs, c->d_stats, sizeof hs, hipMemcpyDeviceToHost, stream));
+    GRT_HIP(c, hipStreamSynchronize(stream));
     if (!exact_schedule && live == 0) {
         // The reference loop ends at the first output step at or after the
         // step on which the last photon terminated.
