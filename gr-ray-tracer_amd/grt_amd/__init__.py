@@ -102,8 +102,8 @@ def load():
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
-        raise GrtError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
-                       "or python gr-ray-tracer_amd/grt_amd/build.py")
+        raise GrtError(f"{LIB_PATH} is missing: build it with "
+                       "python gr-ray-tracer_amd/grt_amd/build.py")
     L = C.CDLL(LIB_PATH)
     P, PH, ST, TU = C.POINTER(Params), C.POINTER(Photons), C.POINTER(Stats), C.POINTER(Tuning)
     vp, dp = C.c_void_p, C.POINTER(C.c_double)
