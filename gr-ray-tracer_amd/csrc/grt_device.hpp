@@ -819,10 +819,14 @@ GRT_HD bool fluid_tetrad(const KSMetric& g, const double u_in[4], T e[16])
 // sample_scattering_fluid, scattering.hpp:185-251.  `g` receives the metric at
 // the photon position (reused by the scattering update).
 // kSep selects the separable fast search at compile time.
-template <int kSep, class T>
+template <int kSep, class T, class OnSample>
 GRT_HD bool sample_fluid(const double s[8], const GridView& G, const StepParams& P,
-                         FluidCellCGS& fl, T e[16], KSMetric& g, uint32_t& cell)
+                         OnSample&& on_sample)
 {
+    FluidCellCGS fl;
+    T e[16];
+    KSMetric g;
+    uint32_t cell;
     if (!G.available) return false;
     const double radius = ks_radius(s[1], s[2], s[3], P.a);
     if (!(radius > 0.0) || !is_finite(radius)) return false;
@@ -859,6 +863,7 @@ GRT_HD bool sample_fluid(const double s[8], const GridView& G, const StepParams&
     fl.magnetic_gauss[1] = (double)inner<T>(g, e + 4, b);
     fl.magnetic_gauss[2] = (double)inner<T>(g, e + 8, b);
     fl.magnetic_gauss[3] = (double)inner<T>(g, e + 12, b);
+    on_sample(fl, e, g, cell);
     return true;
 }
 
@@ -1110,61 +1115,61 @@ GRT_HD bool post_step(Photon& ph, uint8_t phase0, double taken, const GridView& 
     typedef typename std::conditional<kMixed != 0, float, double>::type T;
     const bool image = (P.mode == 0);
     const bool needs = P.scatter || P.absorb || P.emit || (image && phase0 == kBackward);
-    FluidCellCGS fl;
-    T e[16];
-    KSMetric g;
-    uint32_t cell = 0;
-    const bool sampled = needs && sample_fluid<kSep, T>(ph.s, G, P, fl, e, g, cell);
-
-    if (image && phase0 == kBackward && sampled) {
-        double nu = 0.0, energy = 0.0;
-        bool generated = false;
-        const double dcode = fl.density_g_cm3 / P.density_scale;
-        const int packets = ComputeSuperphotonsPerCell(dcode, P.norm);
-        bool defer = false;
-        if (packets > 0) {
-            if (kDefer && P.generator == 0 && !P.scatter && fl.temperature_k > kDeferTmin &&
-                fl.temperature_k < kDeferTmax) {
-                nu = -fl.temperature_k;
-                energy = P.e_cell / (double)packets;
-                rng.draw += kThermalDraws;
-                generated = defer = true;
-            } else {
-                generated = generate_photon(P, fl, rng, packets, nu, energy);
+    // The events run inside the fluid sample's success block (a
+    // continuation), so the tetrad never flows through a merge of the
+    // sample's exit paths.
+    const bool sampled = needs && sample_fluid<kSep, T>(
+        ph.s, G, P, [&](const FluidCellCGS& fl, const T* e, const KSMetric& g, uint32_t cell) {
+            if (image && phase0 == kBackward) {
+                double nu = 0.0, energy = 0.0;
+                bool generated = false;
+                const double dcode = fl.density_g_cm3 / P.density_scale;
+                const int packets = ComputeSuperphotonsPerCell(dcode, P.norm);
+                bool defer = false;
+                if (packets > 0) {
+                    if (kDefer && P.generator == 0 && !P.scatter && fl.temperature_k > kDeferTmin &&
+                        fl.temperature_k < kDeferTmax) {
+                        nu = -fl.temperature_k;
+                        energy = P.e_cell / (double)packets;
+                        rng.draw += kThermalDraws;
+                        generated = defer = true;
+                    } else {
+                        generated = generate_photon(P, fl, rng, packets, nu, energy);
+                    }
+                }
+                if (generated) {
+                    ph.freq = nu;
+                    ph.pend = defer ? 1 : 0;
+                    double eem = 0.0;
+        #pragma unroll
+                    for (int q = 0; q < 4; ++q) eem -= ph.s[4 + q] * (double)e[q];
+                    ph.eem = eem;
+                    ph.stokes[0] = energy;
+                    ph.stokes[1] = ph.stokes[2] = ph.stokes[3] = 0.0;
+                    ph.dl = -fabs(ph.dl);
+                    ph.phase = kForward;
+                } else {
+                    ph.stokes[0] = ph.stokes[1] = ph.stokes[2] = ph.stokes[3] = 0.0;
+                    ph.term = 1;
+                    ph.phase = kRejected;
+                }
+                hook.reset();
+            } else if (phase0 != kBackward) {
+                if (kDefer && ph.pend) resolve_frequency(ph, rng);
+                if (P.absorb || P.emit) hook.rt(fl, cell, ph.freq, taken, ph.stokes[0]);
+                // apply_radiative_transfer_step, radiative_transfer.hpp:81-96
+                if (P.absorb) {
+                    const double dtau = ComputeOpticalDepthIncrement(fl, ph.freq, taken, P.kabs);
+                    const double att = kMixed ? (double)expf((float)-dtau) : exp(-dtau);
+        #pragma unroll
+                    for (int q = 0; q < 4; ++q) ph.stokes[q] *= att;
+                }
+                if (P.emit) ph.stokes[0] += ComputeEmissivityIncrement(fl, ph.freq, taken, P.kemit);
+                if (P.scatter && maybe_scatter<T>(ph.s, ph.stokes, P, fabs(ph.dl), fl, e, g, rng) &&
+                    kMixed)
+                    null_reproject(ph.s, P);
             }
-        }
-        if (generated) {
-            ph.freq = nu;
-            ph.pend = defer ? 1 : 0;
-            double eem = 0.0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) eem -= ph.s[4 + q] * (double)e[q];
-            ph.eem = eem;
-            ph.stokes[0] = energy;
-            ph.stokes[1] = ph.stokes[2] = ph.stokes[3] = 0.0;
-            ph.dl = -fabs(ph.dl);
-            ph.phase = kForward;
-        } else {
-            ph.stokes[0] = ph.stokes[1] = ph.stokes[2] = ph.stokes[3] = 0.0;
-            ph.term = 1;
-            ph.phase = kRejected;
-        }
-        hook.reset();
-    } else if (sampled && phase0 != kBackward) {
-        if (kDefer && ph.pend) resolve_frequency(ph, rng);
-        if (P.absorb || P.emit) hook.rt(fl, cell, ph.freq, taken, ph.stokes[0]);
-        // apply_radiative_transfer_step, radiative_transfer.hpp:81-96
-        if (P.absorb) {
-            const double dtau = ComputeOpticalDepthIncrement(fl, ph.freq, taken, P.kabs);
-            const double att = kMixed ? (double)expf((float)-dtau) : exp(-dtau);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) ph.stokes[q] *= att;
-        }
-        if (P.emit) ph.stokes[0] += ComputeEmissivityIncrement(fl, ph.freq, taken, P.kemit);
-        if (P.scatter && maybe_scatter<T>(ph.s, ph.stokes, P, fabs(ph.dl), fl, e, g, rng) &&
-            kMixed)
-            null_reproject(ph.s, P);
-    }
+        });
 
     if (image && phase0 == kForward) {
         const int crossing = camera_crossing(P, ph.s, au, av);
