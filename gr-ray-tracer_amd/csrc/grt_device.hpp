@@ -5,10 +5,13 @@
 // the host to cross-check the kernel's logic against the oracle.
 //
 // Arithmetic follows the reference's expressions term by term (same
-// operations, same order) and the library is compiled without FMA
-// contraction, so +, -, *, / and sqrt round exactly as in the reference; the
-// step controller's x^(1/4) is correctly rounded (quarter_root).  Citations
-// are file:line in the reference's src/ directory.
+// operations, same order) and the library is compiled without automatic FMA
+// contraction.  The right-hand side and the RK stage sums fuse each product
+// with the sum it joins through explicit fma calls (the rounding of a
+// contracting build of the reference, which the oracle's integrators
+// reproduce); everything else rounds once per +, -, *, / and sqrt.  The step
+// controller's x^(1/4) is correctly rounded (quarter_root).  Citations are
+// file:line in the reference's src/ directory.
 #pragma once
 
 #include <hip/hip_runtime.h>

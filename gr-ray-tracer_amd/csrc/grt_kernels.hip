@@ -2,7 +2,8 @@
 //
 // trace_fp64: the persistent tracing loop (grt_trace.hpp) with the FP64
 // arithmetic of the reference, rounded operation for operation as the
-// oracle rounds it (this file is compiled without FMA contraction).
+// oracle rounds it (this file is compiled without automatic FMA contraction;
+// the only fused operations are the explicit fma calls of grt_device.hpp).
 #include <hip/hip_runtime.h>
 
 #include "grt_internal.hpp"

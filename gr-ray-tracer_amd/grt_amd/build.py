@@ -35,8 +35,9 @@ def hipcc():
 
 
 def compile_flags(resource_usage=False, contract="off"):
-    # -ffp-contract=off: the FP64 path rounds every +, -, * exactly as the
-    # reference's expressions do (no fused multiply-add), see grt_device.hpp.
+    # -ffp-contract=off: the FP64 path fuses only where the source calls fma
+    # explicitly and otherwise rounds every +, -, * as the reference's
+    # expressions do, see grt_device.hpp.
     flags = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=" + contract,
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-function",

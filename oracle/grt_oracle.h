@@ -7,7 +7,9 @@
  *
  * Every function cites the reference file:line it restates (paths relative to
  * the reference's src/ directory).  All arithmetic is IEEE binary64 in the
- * same operation order as the reference (built without FMA contraction).
+ * same operation order as the reference, built without automatic FMA
+ * contraction; the integrators fuse products into sums through explicit fma
+ * calls by default (orc_set_unfused_rounding selects the unfused rounding).
  *
  * RNG boundary: the reference draws from Kokkos::Random_XorShift64_Pool, whose
  * streams cannot be reproduced outside Kokkos.  The oracle plugs in the same
