@@ -21,6 +21,13 @@ using grt::GridView;
 using grt::PhotonPtrs;
 using grt::StepParams;
 
+// Device counters of the tracing driver, followed by the per-trace
+// statistics in the same allocation (kCounterWords keeps them 8-byte
+// aligned), and their pinned host mirror.
+constexpr size_t kCounterWords = 8;
+constexpr size_t kReadbackBytes =
+    kCounterWords * sizeof(uint32_t) + grt::kStatCount * sizeof(unsigned long long);
+
 struct grt_ctx {
     grt_params params{};
     StepParams sp{};
@@ -49,12 +56,18 @@ struct grt_ctx {
     // scratch of the tracing driver
     uint32_t* d_lists = nullptr;  // two lists of list_cap entries
     size_t list_cap = 0;
+    // per-photon scratch, one allocation: rng_cap u32 draw counters, then
+    // rng_cap u8 deferred-frequency flags (cleared together)
     uint32_t* d_rng = nullptr;
-    uint8_t* d_fpend = nullptr;     // deferred thermal frequencies (rng_cap entries)
+    uint8_t* d_fpend = nullptr;
     double* d_fratio = nullptr;
     size_t rng_cap = 0;
-    uint32_t* d_counters = nullptr;  // [0] queue, [1] n_out, [2] live count
+    // one allocation: kCounterWords u32 counters ([0] queue, [1] n_out,
+    // [2] live count, [3] tape overflow, [4] adjoint queue), then the
+    // statistics; read back together into the pinned h_readback
+    uint32_t* d_counters = nullptr;
     unsigned long long* d_stats = nullptr;
+    unsigned char* h_readback = nullptr;
     grt::TraceConsts* d_consts = nullptr;
     grt::TraceConsts h_consts{};   // host source of the copy to d_consts
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -195,15 +208,14 @@ int ensure_scratch(grt_ctx* c, uint64_t n)
     }
     if (n > c->rng_cap) {
         if (c->d_rng) (void)hipFree(c->d_rng);
-        if (c->d_fpend) (void)hipFree(c->d_fpend);
         if (c->d_fratio) (void)hipFree(c->d_fratio);
         c->d_rng = nullptr;
         c->d_fpend = nullptr;
         c->d_fratio = nullptr;
         c->rng_cap = 0;
         const size_t cap = std::max<uint64_t>(n, 1024);
-        GRT_HIP(c, hipMalloc(&c->d_rng, cap * sizeof(uint32_t)));
-        GRT_HIP(c, hipMalloc(&c->d_fpend, cap * sizeof(uint8_t)));
+        GRT_HIP(c, hipMalloc(&c->d_rng, cap * (sizeof(uint32_t) + sizeof(uint8_t))));
+        c->d_fpend = reinterpret_cast<uint8_t*>(c->d_rng + cap);
         GRT_HIP(c, hipMalloc(&c->d_fratio, cap * sizeof(double)));
         c->rng_cap = cap;
     }
@@ -348,13 +360,14 @@ int grt_create(const grt_params* p, grt_ctx** out)
     c->max_lds = prop.sharedMemPerBlock;
     c->params = *p;
     c->sp = grt::make_step_params(*p);
-    if (hipMalloc(&c->d_counters, 8 * sizeof(uint32_t)) != hipSuccess ||
-        hipMalloc(&c->d_stats, grt::kStatCount * sizeof(unsigned long long)) != hipSuccess ||
+    if (hipMalloc(&c->d_counters, kReadbackBytes) != hipSuccess ||
+        hipHostMalloc(&c->h_readback, kReadbackBytes) != hipSuccess ||
         hipMalloc(&c->d_consts, sizeof(grt::TraceConsts)) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
         grt_destroy(c);
         return fail(nullptr, GRT_E_NOMEM, "device allocation failed in grt_create");
     }
+    c->d_stats = reinterpret_cast<unsigned long long*>(c->d_counters + kCounterWords);
     *out = c;
     return GRT_OK;
 }
@@ -363,10 +376,11 @@ int grt_destroy(grt_ctx* c)
 {
     if (!c) return GRT_OK;
     free_grid(c);
-    void* bufs[] = {c->d_lists, c->d_rng, c->d_fpend, c->d_fratio, c->d_counters, c->d_stats,
-                    c->d_stage, c->d_img, c->d_consts, c->d_tape, c->d_thdr, c->d_rec};
+    void* bufs[] = {c->d_lists, c->d_rng, c->d_fratio, c->d_counters, c->d_stage, c->d_img,
+                    c->d_consts, c->d_tape, c->d_thdr, c->d_rec};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    if (c->h_readback) (void)hipHostFree(c->h_readback);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     delete c;
@@ -492,10 +506,9 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
     P.scatter = P.scatter && fluid;
 
     grt_stats st{};
-    GRT_HIP(c, hipMemsetAsync(c->d_rng, 0, n * sizeof(uint32_t), stream));
-    GRT_HIP(c, hipMemsetAsync(c->d_fpend, 0, n * sizeof(uint8_t), stream));
-    GRT_HIP(c, hipMemsetAsync(c->d_stats, 0, grt::kStatCount * sizeof(unsigned long long), stream));
-    GRT_HIP(c, hipMemsetAsync(c->d_counters, 0, 8 * sizeof(uint32_t), stream));
+    // draw counters and deferred-frequency flags; counters and statistics
+    GRT_HIP(c, hipMemsetAsync(c->d_rng, 0, c->rng_cap * sizeof(uint32_t) + n, stream));
+    GRT_HIP(c, hipMemsetAsync(c->d_counters, 0, kReadbackBytes, stream));
     uint32_t* list[2] = {c->d_lists, c->d_lists + c->list_cap};
     GRT_HIP(c, grt::launch_build_live(dev->terminate, n, list[0], c->d_counters + 2, stream));
     uint32_t live = 0;
@@ -589,9 +602,17 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
             GRT_HIP(c, hipEventRecord(c->ev0, stream));
             GRT_HIP(c, grt::launch_trace(L, blocks, block, lds_bytes, stream));
             GRT_HIP(c, hipEventRecord(c->ev1, stream));
-            GRT_HIP(c, hipMemcpyAsync(&live, c->d_counters + 1, sizeof(uint32_t),
+            // Thermal frequencies whose draw the tracing kernel deferred.  A
+            // photon still in flight continues with its drawn frequency,
+            // which the kernel would have drawn the same way later; queued
+            // before the synchronisation, the last launch needs no second
+            // round trip.  Counters and statistics come back in one copy.
+            GRT_HIP(c, grt::launch_resolve_deferred(c->h_consts.ph, n, P.seed, P.stream << 1,
+                                                    stream));
+            GRT_HIP(c, hipMemcpyAsync(c->h_readback, c->d_counters, kReadbackBytes,
                                       hipMemcpyDeviceToHost, stream));
             GRT_HIP(c, hipStreamSynchronize(stream));
+            live = reinterpret_cast<const uint32_t*>(c->h_readback)[1];
             float ms = 0.0f;
             GRT_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
             kernel_ms += ms;
@@ -606,11 +627,6 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
         if (fraction >= termination_percent) break;
         if (!exact_schedule && live == 0) break;
     }
-    if (n > 0) {
-        // Thermal frequencies whose draw the tracing kernel deferred (stream
-        // ordered; the statistics copy below waits for it).
-        GRT_HIP(c, grt::launch_resolve_deferred(c->h_consts.ph, n, P.seed, P.stream << 1, stream));
-    }
     if (tape_slots > 0) {
         GRT_HIP(c, grt::launch_tape_finalize(c->d_thdr, n, tape_slots, c->d_counters + 3, stream));
         uint32_t overflow = 0;
@@ -624,9 +640,13 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
         c->tape_slots = tape_slots;
         c->tape_integrator = P.integrator;
     }
-    unsigned long long hs[grt::kStatCount];
-    GRT_HIP(c, hipMemcpyAsync(hs, c->d_stats, sizeof hs, hipMemcpyDeviceToHost, stream));
-    GRT_HIP(c, hipStreamSynchronize(stream));
+    if (launches == 0) {
+        GRT_HIP(c, hipMemcpyAsync(c->h_readback, c->d_counters, kReadbackBytes,
+                                  hipMemcpyDeviceToHost, stream));
+        GRT_HIP(c, hipStreamSynchronize(stream));
+    }
+    const unsigned long long* hs =
+        reinterpret_cast<const unsigned long long*>(c->h_readback + kCounterWords * sizeof(uint32_t));
     if (!exact_schedule && live == 0) {
         // The reference loop ends at the first output step at or after the
         // step on which the last photon terminated.
