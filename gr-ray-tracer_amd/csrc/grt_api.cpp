@@ -511,9 +511,9 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
     GRT_HIP(c, hipMemsetAsync(c->d_counters, 0, kReadbackBytes, stream));
     uint32_t* list[2] = {c->d_lists, c->d_lists + c->list_cap};
     GRT_HIP(c, grt::launch_build_live(dev->terminate, n, list[0], c->d_counters + 2, stream));
-    uint32_t live = 0;
-    GRT_HIP(c, hipMemcpyAsync(&live, c->d_counters + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    GRT_HIP(c, hipStreamSynchronize(stream));
+    // The first launch reads the live count on the device; until its
+    // readback, n bounds it.
+    uint32_t live = (uint32_t)n;
 
     c->h_consts.ph = to_ptrs(*dev, c->d_rng);
     c->h_consts.ph.fpend = c->d_fpend;
@@ -590,10 +590,14 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
         while (left > 0 && live > 0) {
             const int budget = (int)std::min<int64_t>(left, c->tune.sub_budget);
             // With every photon live the list is a permutation of all
-            // photons, and the order of processing does not change results.
-            L.list_in = (launches == 0 && live == n) ? nullptr : list[cur];
+            // photons, and the order of processing does not change results:
+            // the first launch then walks the photons directly (the kernel
+            // decides, from the count build_live left on the device).
+            L.list_in = list[cur];
             L.list_out = list[cur ^ 1];
             L.n_in = live;
+            L.n_in_dev = launches == 0 ? c->d_counters + 2 : nullptr;
+            L.n_total = (uint32_t)n;
             L.budget = budget;
             L.step0 = (int)step0;
             GRT_HIP(c, hipMemsetAsync(c->d_counters, 0, 2 * sizeof(uint32_t), stream));

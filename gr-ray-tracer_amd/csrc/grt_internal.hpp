@@ -82,6 +82,11 @@ struct TraceLaunch {
     int integrator;
     const uint32_t* list_in;  // nullptr: the identity list (slot i is photon i)
     uint32_t n_in;
+    // When set, the kernel reads n_in from here (the live count build_live
+    // wrote just before, so the host need not wait for it) and walks the
+    // identity list instead of list_in if all n_total photons are live.
+    const uint32_t* n_in_dev;
+    uint32_t n_total;
     uint32_t* queue;          // work counter over list_in, zeroed before launch
     uint32_t* list_out;       // photons still live after `budget` calls
     uint32_t* n_out;          // zeroed before launch

@@ -245,9 +245,14 @@ __device__ __forceinline__ uint64_t tape_end_call(uint64_t h, bool touched, bool
 // the arithmetic of the step (see lane_iterate), kTape the geodesic
 // checkpoints of the differentiable form.
 template <int kIntegrator, int kSep, class Rhs, int kMixed, int kTape = 0>
-__device__ __forceinline__ void trace_body(const TraceLaunch& a)
+__device__ __forceinline__ void trace_body(const TraceLaunch& a_in)
 {
     extern __shared__ double lds[];
+    TraceLaunch a = a_in;
+    if (a_in.n_in_dev) {
+        a.n_in = *a_in.n_in_dev;
+        if (a.n_in == a_in.n_total) a.list_in = nullptr;
+    }
     GridView G = a.grid;
     if (kSep) {
         // Stage the separable search tables in LDS (they always fit: larger
