@@ -178,6 +178,23 @@ def test_empty_batches(gpu, c1):
     assert "2^31" in ctx._L.grt_last_error(ctx._h).decode()
 
 
+def test_all_terminated_on_entry(oracle, gpu, c1):
+    """Photons all terminated on entry: the first launch reads a live count of
+    zero on the device and leaves every photon untouched, as the reference's
+    functor returns at once for a terminated photon."""
+    import grt_amd
+    kw = c1_params()
+    su, sv = pixel_screen(16)
+    ctx = gpu_ctx(kw, c1)
+    po, G = oracle_parts(oracle, kw, c1)
+    ph0 = oracle.init_image(po, su, sv, 0.1)
+    ph0["terminate"][:] = 1
+    g, o, st_g, st_o = trace_both(oracle, ctx, po, G, ph0, max_steps=50)
+    assert st_g["functor_calls"] == 0 and st_g["terminated"] == len(su)
+    for k in ("x0", "x1", "k1", "I", "frequency", "dlambda", "phase"):
+        assert np.array_equal(g[k], ph0[k]) and np.array_equal(o[k], ph0[k]), k
+
+
 def test_rk4_general_grid_and_capture(oracle, gpu):
     """RK4, the non-separable search path and horizon capture together."""
     from grt_amd.fm_torus import fishbone_moncrief
