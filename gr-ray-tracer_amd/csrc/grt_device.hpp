@@ -819,8 +819,10 @@ GRT_HD bool fluid_tetrad(const KSMetric& g, const double u_in[4], T e[16])
     return true;
 }
 
-// sample_scattering_fluid, scattering.hpp:185-251.  `g` receives the metric at
-// the photon position (reused by the scattering update).
+// sample_scattering_fluid, scattering.hpp:185-251.  On success it calls
+// on_sample(fluid, tetrad e, metric g at the photon position (reused by the
+// scattering update), cell index) and returns true; the caller's events run
+// inside that call, where the tetrad is defined on every path.
 // kSep selects the separable fast search at compile time.
 template <int kSep, class T, class OnSample>
 GRT_HD bool sample_fluid(const double s[8], const GridView& G, const StepParams& P,
