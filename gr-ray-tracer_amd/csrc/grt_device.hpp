@@ -1108,7 +1108,7 @@ GRT_HD void null_reproject(double s[8], const StepParams& P)
 // kDefer: the image-mode emission of a thermal packet (generator 0, no
 // scattering) only reserves its five draws and keeps -T in freq (pend set);
 // the frequency is drawn when a transfer update first needs it, or after the
-// trace (resolve_deferred), where every photon draws it at once instead of
+// launch (resolve_deferred), where every photon draws it at once instead of
 // one or two lanes at a time.  On arrival the redshift factor eobs / eem is
 // returned in fr for the deferred photon.  Results are bit for bit those of
 // the immediate emission.

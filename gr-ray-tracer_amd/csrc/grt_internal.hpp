@@ -145,7 +145,7 @@ constexpr int kRecEntries = 64;    // replay records per lane: <= 2 per call
 constexpr int kRecDoubles = 16;    // cell (or -1: reset), I before, step, frequency, fluid[12]
 hipError_t launch_build_live(const uint8_t* term, uint64_t n, uint32_t* list, uint32_t* count,
                              hipStream_t stream);
-// Draws the deferred thermal frequencies after a trace (grt_kernels.hip).
+// Draws the deferred thermal frequencies after a launch (grt_kernels.hip).
 hipError_t launch_resolve_deferred(const PhotonPtrs& ph, uint64_t n, uint64_t seed,
                                    uint32_t stream_id, hipStream_t stream);
 hipError_t launch_image_products(const PhotonPtrs& ph, uint64_t n, int64_t nx, int64_t ny,

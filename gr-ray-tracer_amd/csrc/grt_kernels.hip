@@ -74,9 +74,10 @@ __device__ __forceinline__ int64_t obs_bin(double v, double lo, double hi, int64
 constexpr int kLdsBins = 4096;
 
 // build_observation_products, observation.hpp:31-84 (sums by atomics).
-// Draws the thermal frequencies whose draw the tracing kernel deferred: the
-// five draws before the photon's final draw counter, at T = -freq, then the
-// arrival redshift factor for a photon that reached the camera.
+// Draws the thermal frequencies whose draw a tracing launch deferred: the
+// five draws before the photon's saved draw counter, at T = -freq, then the
+// arrival redshift factor for a photon that reached the camera.  A photon
+// still in flight continues with the drawn frequency (fpend cleared).
 __global__ __launch_bounds__(256) void resolve_deferred(PhotonPtrs ph, uint64_t n, uint64_t seed,
                                                         uint32_t stream)
 {
