@@ -178,6 +178,32 @@ def test_empty_batches(gpu, c1):
     assert "2^31" in ctx._L.grt_last_error(ctx._h).decode()
 
 
+def test_precision_workload_geometry_128(oracle, gpu):
+    """The precision workload's configuration (FM torus a=0.94 without a
+    density floor on 256x128x128, 17 deg, RK45 rtol 1e-7) at 128^2 rays:
+    positions, momenta, step sizes and counters bit-identical to the oracle;
+    Stokes I and the image to the last ulp of exp (1e-14)."""
+    spin = 0.94
+    fields = c1_fields(rho_floor=0.0, nr=256, nth=128, nph=128, spin=spin)
+    kw = c1_params(spin=spin, theta=np.radians(17.0), atol=1e-7, rtol=1e-7,
+                   superphoton_count_normalization=1e9)
+    su, sv = pixel_screen(128)
+    ctx = gpu_ctx(kw, fields)
+    po, G = oracle_parts(oracle, kw, fields)
+    ph0 = oracle.init_image(po, su, sv, 0.1)
+    g, o, st_g, st_o = trace_both(oracle, ctx, po, G, ph0)
+    for k in ("x0", "x1", "x2", "x3", "k0", "k1", "k2", "k3", "dlambda", "phase", "terminate",
+              "theta_disp", "phi_disp"):
+        assert np.array_equal(g[k], o[k], equal_nan=True), k
+    for k in ("functor_calls", "accepted_steps", "rejected_attempts", "fluid_samples",
+              "failed_steps", "global_steps", "terminated"):
+        assert st_g[k] == st_o[k], k
+    assert np.allclose(g["I"], o["I"], rtol=1e-14, atol=0.0)
+    img_g = ctx.image_products(g, 128, 128)
+    img_o = oracle.image_products(o, 128, 128, 128, 1e8, 1e22, 50.0, 50.0)
+    assert np.allclose(img_g["image"], img_o["image"], rtol=1e-14, atol=0.0)
+
+
 def test_all_terminated_on_entry(oracle, gpu, c1):
     """Photons all terminated on entry: the first launch reads a live count of
     zero on the device and leaves every photon untouched, as the reference's
