@@ -55,14 +55,16 @@ struct StepParams {
 // Kerr-Schild geometry, metrics/kerr_schild_core.hpp
 // ---------------------------------------------------------------------------
 
+GRT_HD double sqrt_ieee(double x);   // IEEE sqrt, defined with the exact sequences below
+
 // kerr_schild_core.hpp:28-33
 GRT_HD double ks_radius(double x, double y, double z, double a)
 {
     const double rsq = x * x + y * y + z * z;
     const double A = rsq - a * a;
     const double az = a * z;
-    const double S = sqrt(A * A + 4.0 * (az * az));
-    return sqrt(0.5 * (A + S));
+    const double S = sqrt_ieee(A * A + 4.0 * (az * az));
+    return sqrt_ieee(0.5 * (A + S));
 }
 
 // kerr_schild_core.hpp:80-83 with l_0 = 1: the g_00 entry, 2 H l_0 l_0 - 1.
@@ -151,6 +153,26 @@ GRT_HD double rcp_rn(double d)
 #else
     return 1.0 / d;
 #endif
+}
+
+// sqrt(x) and 1.0 / sqrt(x) with the IEEE results: the short sequences of
+// sqrt_rn / rcp_rn inside their exact range, the library ones outside it
+// (zero, denormal, huge or non-finite arguments).  A file compiled with
+// relaxed math (GRT_RELAXED_MATH) keeps the plain library calls.
+GRT_HD double sqrt_ieee(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(GRT_RELAXED_MATH)
+    if (x > 0x1p-600 && x < 0x1p600) return sqrt_rn(x);
+#endif
+    return sqrt(x);
+}
+
+GRT_HD double rsqrt_ieee(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(GRT_RELAXED_MATH)
+    if (x > 0x1p-600 && x < 0x1p600) return rcp_rn(sqrt_rn(x));
+#endif
+    return 1.0 / sqrt(x);
 }
 
 // Hamilton's equations for H = 1/2 g^{mu nu} p_mu p_nu in Cartesian
@@ -758,12 +780,16 @@ struct KSMetric {
     double twoH, l1, l2, l3;
 };
 
-GRT_HD KSMetric ks_metric_at(const double s[8], double a, double m)
+GRT_HD KSMetric ks_metric_at(const double s[8], double r, double a, double m)
 {
-    const double r = ks_radius(s[1], s[2], s[3], a);
     double l[4];
     ks_l(s[1], s[2], s[3], r, a, l);
     return KSMetric{2.0 * ks_H(r, s[3], a, m), l[1], l[2], l[3]};
+}
+
+GRT_HD KSMetric ks_metric_at(const double s[8], double a, double m)
+{
+    return ks_metric_at(s, ks_radius(s[1], s[2], s[3], a), a, m);
 }
 
 // metric_inner_product (fluid_frame.hpp:6-14) through the factored metric:
@@ -793,7 +819,9 @@ GRT_HD bool fluid_tetrad(const KSMetric& g, const double u_in[4], T e[16])
     const T u[4] = {(T)u_in[0], (T)u_in[1], (T)u_in[2], (T)u_in[3]};
     const T vn = inner<T>(g, u, u);
     if (!(vn < (T)0) || !is_finite((double)vn)) return false;
-    const T inv = (T)1 / sqrt(-vn);
+    T inv;
+    if constexpr (std::is_same<T, double>::value) inv = rsqrt_ieee(-vn);
+    else inv = (T)1 / sqrt(-vn);
     const T sgn = u[0] >= (T)0 ? (T)1 : (T)-1;
 #pragma unroll
     for (int mu = 0; mu < 4; ++mu) e[mu] = sgn * u[mu] * inv;
@@ -812,7 +840,9 @@ GRT_HD bool fluid_tetrad(const KSMetric& g, const double u_in[4], T e[16])
         }
         const T nn = inner<T>(g, c, c);
         if (!((double)nn > 1.0e-20) || !is_finite((double)nn)) return false;
-        const T is = (T)1 / sqrt(nn);
+        T is;
+        if constexpr (std::is_same<T, double>::value) is = rsqrt_ieee(nn);
+        else is = (T)1 / sqrt(nn);
 #pragma unroll
         for (int mu = 0; mu < 4; ++mu) e[4 * (ax + 1) + mu] = c[mu] * is;
     }
@@ -861,7 +891,7 @@ GRT_HD bool sample_fluid(const double s[8], const GridView& G, const StepParams&
         return false;
 #pragma unroll
     for (int q = 0; q < 4; ++q) fl.coordinate_four_velocity_cm_s[q] = c[2 + q];
-    g = ks_metric_at(s, P.a, P.m);
+    g = ks_metric_at(s, radius, P.a, P.m);
     if (!fluid_tetrad<T>(g, fl.coordinate_four_velocity_cm_s, e)) return false;
     const double b[4] = {c[6], c[7], c[8], c[9]};
     fl.magnetic_gauss[0] = -(double)inner<T>(g, e, b);
@@ -1021,7 +1051,7 @@ GRT_HD double observer_energy(const StepParams& P, const double s[8])
     const double r = ks_radius(s[1], s[2], s[3], P.a);
     const double g00 = 2.0 * ks_H(r, s[3], P.a, P.m) + (-1.0);
     if (!(g00 < 0.0)) return 0.0;
-    return -s[4] * (1.0 / sqrt(-g00));
+    return -s[4] * rsqrt_ieee(-g00);
 }
 
 // ---------------------------------------------------------------------------

@@ -26,6 +26,8 @@
 // replaces FP64 division, square roots and transcendentals.
 #include <hip/hip_runtime.h>
 
+// The library's sqrt and 1/sqrt, not the exact short sequences (grt_device.hpp).
+#define GRT_RELAXED_MATH 1
 #include "grt_internal.hpp"
 #include "grt_trace.hpp"
 
