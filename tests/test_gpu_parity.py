@@ -280,6 +280,10 @@ def test_trace_resumes_after_stop(oracle, gpu, c1):
 
 
 def test_scattering_image(oracle, gpu, c1):
+    """Image mode with scattering enabled: the forward trace retraces the
+    backward path and meets no fluid, so no scattering fires (the reference's
+    algorithm); GPU and oracle agree.  Scattering itself is tested in disk
+    mode (test_disk_init_and_scattering_statistics)."""
     kw = c1_params(scattering_enabled=1, albedo=0.7, seed=99, length_cm_per_code=1e24)
     su, sv = pixel_screen(24)
     ctx = gpu_ctx(kw, c1)

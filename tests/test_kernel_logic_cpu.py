@@ -1,8 +1,9 @@
 """The tracing kernel's per-lane logic (grt_device.hpp, compiled for the CPU
 by tests/harness) against the oracle on the first reference configuration
 and on variants exercising every branch of the functor: RK4, the general
-(non-separable) cell search, scattering (same Philox stream, so per-photon
-exact), disk-mode packets, horizon capture, 12-failure termination.
+(non-separable) cell search, image mode with scattering enabled (where it
+never fires), disk-mode packets with multiple scattering (statistical),
+horizon capture, 12-failure termination.
 
 Compiled for the CPU (the only fused operations are the explicit fma calls
 that the oracle makes too) the kernel logic reproduces the oracle to the last
@@ -91,13 +92,24 @@ def test_torus_image_rk4_and_horizon(oracle):
     assert counts[4] > 0 and counts[3] > 0
 
 
-def test_scattering_per_photon_exact(oracle, c1):
-    kw = c1_params(scattering_enabled=1, albedo=0.7, seed=99,
-                   length_cm_per_code=1e24)   # optical depth of order one per step
-    su, sv = pixel_screen(24)
-    ph0 = oracle.init_image(oracle.make_params(**split_units(kw)[0]), su, sv, 0.1)
-    A, B, st_o, st_h = run(oracle, c1, kw, ph0, max_steps=2000)
-    assert_same(A, B, st_o, st_h)
+def test_image_mode_scattering_never_fires(oracle, c1):
+    """In image mode the forward trace after the emission event retraces the
+    backward path, which met no fluid before the emission point, so the
+    forward phase never samples fluid: transfer and scattering after emission
+    never fire (integrate_cart_ks.hpp:323-376), whatever the optical depth.
+    Kernel logic and oracle agree bit for bit, and the albedo changes
+    nothing."""
+    out = []
+    for albedo in (0.7, 0.2):
+        kw = c1_params(scattering_enabled=1, albedo=albedo, seed=99,
+                       length_cm_per_code=1e24)   # optical depth >> 1 per step in fluid
+        su, sv = pixel_screen(24)
+        ph0 = oracle.init_image(oracle.make_params(**split_units(kw)[0]), su, sv, 0.1)
+        A, B, st_o, st_h = run(oracle, c1, kw, ph0, max_steps=2000)
+        assert_same(A, B, st_o, st_h)
+        out.append(A)
+    for k in ("k1", "k2", "k3", "I", "phase"):
+        assert np.array_equal(out[0][k], out[1][k]), k
 
 
 def disk_packets(oracle, fields, kw):
