@@ -801,9 +801,17 @@ GRT_HD T inner(const KSMetric& g, const U u[4], const V v[4])
     const T l1 = (T)g.l1, l2 = (T)g.l2, l3 = (T)g.l3;
     const T u0 = (T)u[0], u1 = (T)u[1], u2 = (T)u[2], u3 = (T)u[3];
     const T v0 = (T)v[0], v1 = (T)v[1], v2 = (T)v[2], v3 = (T)v[3];
-    const T lu = u0 + l1 * u1 + l2 * u2 + l3 * u3;
-    const T lv = v0 + l1 * v1 + l2 * v2 + l3 * v3;
-    return (-u0 * v0 + u1 * v1 + u2 * v2 + u3 * v3) + (T)g.twoH * lu * lv;
+    if constexpr (std::is_same<T, double>::value) {
+        // fused: each product joins its sum in one fma
+        const double lu = fma(l3, u3, fma(l2, u2, fma(l1, u1, u0)));
+        const double lv = fma(l3, v3, fma(l2, v2, fma(l1, v1, v0)));
+        const double eta = fma(u3, v3, fma(u2, v2, fma(u1, v1, -u0 * v0)));
+        return fma(g.twoH * lu, lv, eta);
+    } else {
+        const T lu = u0 + l1 * u1 + l2 * u2 + l3 * u3;
+        const T lv = v0 + l1 * v1 + l2 * v2 + l3 * v3;
+        return (-u0 * v0 + u1 * v1 + u2 * v2 + u3 * v3) + (T)g.twoH * lu * lv;
+    }
 }
 
 GRT_HD double inner(const KSMetric& g, const double u[4], const double v[4])
