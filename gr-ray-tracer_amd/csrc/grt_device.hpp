@@ -821,8 +821,13 @@ GRT_HD double inner(const KSMetric& g, const double u[4], const double v[4])
 
 // build_fluid_tetrad, fluid_frame.hpp:52-80: e[0] = u / |u|, then
 // Gram-Schmidt of the coordinate x, y, z axes under g, in arithmetic T.
+// full = false stops after e[0] and returns what the whole construction
+// would: with u timelike (the check that gives e[0]) and g = eta + 2H l l
+// with H >= 0, the coordinate axes span a positive-definite subspace whose
+// metric is at least the identity, so every Gram-Schmidt norm is >= 1 and
+// its check (> 1e-20, finite) cannot fail.
 template <class T>
-GRT_HD bool fluid_tetrad(const KSMetric& g, const double u_in[4], T e[16])
+GRT_HD bool fluid_tetrad(const KSMetric& g, const double u_in[4], T e[16], bool full = true)
 {
     const T u[4] = {(T)u_in[0], (T)u_in[1], (T)u_in[2], (T)u_in[3]};
     const T vn = inner<T>(g, u, u);
@@ -833,6 +838,7 @@ GRT_HD bool fluid_tetrad(const KSMetric& g, const double u_in[4], T e[16])
     const T sgn = u[0] >= (T)0 ? (T)1 : (T)-1;
 #pragma unroll
     for (int mu = 0; mu < 4; ++mu) e[mu] = sgn * u[mu] * inv;
+    if (!full) return true;
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
         T c[4] = {(T)0, (T)0, (T)0, (T)0};
@@ -864,7 +870,7 @@ GRT_HD bool fluid_tetrad(const KSMetric& g, const double u_in[4], T e[16])
 // kSep selects the separable fast search at compile time.
 template <int kSep, class T, class OnSample>
 GRT_HD bool sample_fluid(const double s[8], const GridView& G, const StepParams& P,
-                         OnSample&& on_sample)
+                         bool full, OnSample&& on_sample)
 {
     FluidCellCGS fl;
     T e[16];
@@ -900,12 +906,14 @@ GRT_HD bool sample_fluid(const double s[8], const GridView& G, const StepParams&
 #pragma unroll
     for (int q = 0; q < 4; ++q) fl.coordinate_four_velocity_cm_s[q] = c[2 + q];
     g = ks_metric_at(s, radius, P.a, P.m);
-    if (!fluid_tetrad<T>(g, fl.coordinate_four_velocity_cm_s, e)) return false;
-    const double b[4] = {c[6], c[7], c[8], c[9]};
-    fl.magnetic_gauss[0] = -(double)inner<T>(g, e, b);
-    fl.magnetic_gauss[1] = (double)inner<T>(g, e + 4, b);
-    fl.magnetic_gauss[2] = (double)inner<T>(g, e + 8, b);
-    fl.magnetic_gauss[3] = (double)inner<T>(g, e + 12, b);
+    if (!fluid_tetrad<T>(g, fl.coordinate_four_velocity_cm_s, e, full)) return false;
+    if (full) {
+        const double b[4] = {c[6], c[7], c[8], c[9]};
+        fl.magnetic_gauss[0] = -(double)inner<T>(g, e, b);
+        fl.magnetic_gauss[1] = (double)inner<T>(g, e + 4, b);
+        fl.magnetic_gauss[2] = (double)inner<T>(g, e + 8, b);
+        fl.magnetic_gauss[3] = (double)inner<T>(g, e + 12, b);
+    }
     on_sample(fl, e, g, cell);
     return true;
 }
@@ -1161,8 +1169,15 @@ GRT_HD bool post_step(Photon& ph, uint8_t phase0, double taken, const GridView& 
     // The events run inside the fluid sample's success block (a
     // continuation), so the tetrad never flows through a merge of the
     // sample's exit paths.
+    // The image-mode emission event reads only e[0] (the emission-frame
+    // energy) and, for the thermal and power-law generators, nothing of the
+    // magnetic field: the rest of the tetrad is skipped there (every lane
+    // emits once, so the wavefront would otherwise pay for it on most
+    // iterations with one or two lanes active).
+    const bool full = !(image && phase0 == kBackward) || P.generator == 2;
     const bool sampled = needs && sample_fluid<kSep, T>(
-        ph.s, G, P, [&](const FluidCellCGS& fl, const T* e, const KSMetric& g, uint32_t cell) {
+        ph.s, G, P, full,
+        [&](const FluidCellCGS& fl, const T* e, const KSMetric& g, uint32_t cell) {
             if (image && phase0 == kBackward) {
                 double nu = 0.0, energy = 0.0;
                 bool generated = false;
