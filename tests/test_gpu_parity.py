@@ -120,6 +120,22 @@ def test_c1_image_64(oracle, gpu, c1):
         assert st_g[k] == st_o[k], k
 
 
+@pytest.mark.parametrize("generator", [1, 2])
+def test_image_generators(oracle, gpu, c1, generator):
+    """Image mode with the power-law generator (the emission event reads only
+    the first tetrad vector) and the custom generator (it reads the whole
+    fluid cell, magnetic field included): every photon as the oracle."""
+    kw = c1_params(generator=generator, nu_min_hz=1e9, nu_max_hz=1e12)
+    su, sv = pixel_screen(32)
+    ctx = gpu_ctx(kw, c1)
+    po, G = oracle_parts(oracle, kw, c1)
+    ph0 = oracle.init_image(po, su, sv, 0.1)
+    g, o, st_g, st_o = trace_both(oracle, ctx, po, G, ph0)
+    check_states(g, o)
+    check_image(ctx, oracle, g, o, 32)
+    assert st_g["fluid_samples"] == st_o["fluid_samples"]
+
+
 def test_work_list_edges(oracle, gpu, c1):
     """The persistent loop's work list at its edges: batches smaller than,
     equal to and just above one wavefront, photons already terminated on

@@ -149,6 +149,19 @@ def test_disk_mode_multiple_scattering(oracle):
     assert abs(st_o["accepted_steps"] - st_h["accepted_steps"]) < 0.02 * st_o["accepted_steps"]
 
 
+@pytest.mark.parametrize("generator", [1, 2])
+def test_image_mode_generators(oracle, c1, generator):
+    """Image mode with the power-law and the custom generator
+    (photon_generation.hpp:99-156): the first reads only the emission-frame
+    energy of the fluid tetrad, the second the whole sampled cell."""
+    kw = c1_params(generator=generator, nu_min_hz=1e9, nu_max_hz=1e12)
+    su, sv = pixel_screen(12)
+    ph0 = oracle.init_image(oracle.make_params(**kw), su, sv, 0.1)
+    A, B, st_o, st_h = run(oracle, c1, kw, ph0, max_steps=3000)
+    assert_same(A, B, st_o, st_h)
+    assert (A["phase"] == 3).sum() > 10 and np.count_nonzero(A["frequency"]) > 10
+
+
 def test_twelve_failures_terminate(oracle, c1):
     """Twelve rejected attempts in one functor call terminate the photon with
     its phase unchanged (integrate_cart_ks.hpp:268-299).  A non-finite
