@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256, GRT_TRACE_WAVES_PER_EU) void adjoint_fp64(Adjo
             lane_reset(L);
             const uint32_t ncalls = min(kSegmentCalls, calls_total - call0);
             h.count = 0;
-            LaneCounters cnt{0u, 0u, 0u, 0u, 0u};
+            LaneCounters cnt{0u, 0u, 0u, 0u, 0u, 0u};
             uint32_t done = 0;
             while (done < ncalls) {
                 double au = 0.0, av = 0.0, fr = 1.0;

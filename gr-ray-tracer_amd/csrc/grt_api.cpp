@@ -39,6 +39,11 @@ struct grt_ctx {
 
     // grid
     bool has_grid = false;
+    // The cell records fold in the spin (the spherical-to-Cartesian Jacobian)
+    // and the unit scales of the params current at grt_set_grid; a later
+    // grt_set_params that changes one of them leaves the records stale until
+    // the grid is set again.
+    bool grid_stale = false;
     int nr = 0, nth = 0, nph = 0;
     bool separable = false;
     double* d_tables = nullptr;   // separable search tables, then the three axes
@@ -78,7 +83,9 @@ struct grt_ctx {
     size_t tape_cap = 0;           // doubles
     uint64_t* d_thdr = nullptr;
     size_t thdr_cap = 0;
-    uint64_t tape_n = 0;           // photons of the last taped trace (0: none)
+    uint64_t tape_n = 0;           // photons of the last taped trace (0: none, or
+                                   // invalidated by a later change of grid,
+                                   // fluid or geometry)
     int tape_slots = 0;
     int tape_integrator = 0;
     double* d_rec = nullptr;
@@ -178,6 +185,38 @@ int validate_params(grt_ctx* ctx, const grt_params* p)
     for (double v : u)
         if (!(v > 0.0) || !finite(v))
             return fail(ctx, GRT_E_INVALID, "all [units] conversion factors must be finite and positive");
+    return GRT_OK;
+}
+
+// Whether two parameter sets differ at most in the two transfer coefficients
+// (field by field: the struct has padding).
+bool same_except_coefficients(const grt_params& a, const grt_params& b)
+{
+#define GRT_SAME(f) (a.f == b.f)
+    return GRT_SAME(a_bh) && GRT_SAME(m_bh) && GRT_SAME(r_term_min) && GRT_SAME(r_term_max) &&
+           GRT_SAME(atol) && GRT_SAME(rtol) && GRT_SAME(min_step_scale) &&
+           GRT_SAME(max_step_scale) && GRT_SAME(safety) && GRT_SAME(integrator) &&
+           GRT_SAME(mode) && GRT_SAME(absorption_enabled) && GRT_SAME(emission_enabled) &&
+           GRT_SAME(scattering_enabled) && GRT_SAME(albedo) && GRT_SAME(seed) &&
+           GRT_SAME(rng_stream) && GRT_SAME(generator) &&
+           GRT_SAME(superphoton_count_normalization) && GRT_SAME(energy_per_cell_erg) &&
+           GRT_SAME(power_law_slope) && GRT_SAME(nu_min_hz) && GRT_SAME(nu_max_hz) &&
+           GRT_SAME(custom_frequency_hz) && GRT_SAME(camera_distance) &&
+           GRT_SAME(camera_theta) && GRT_SAME(camera_phi) && GRT_SAME(camera_width) &&
+           GRT_SAME(camera_height) && GRT_SAME(length_cm_per_code) &&
+           GRT_SAME(time_s_per_code) && GRT_SAME(density_g_cm3_per_code) &&
+           GRT_SAME(temperature_k_per_code) && GRT_SAME(four_velocity_cm_s_per_code) &&
+           GRT_SAME(magnetic_gauss_per_code) && GRT_SAME(precision);
+#undef GRT_SAME
+}
+
+// Fluid-dependent entry points refuse a grid whose cell records no longer
+// match the params.
+int check_grid_fresh(grt_ctx* c)
+{
+    if (c->has_grid && c->grid_stale)
+        return fail(c, GRT_E_STATE, "black_hole.spin or a [units] scale changed after "
+                                    "grt_set_grid; call grt_set_grid again");
     return GRT_OK;
 }
 
@@ -392,6 +431,16 @@ int grt_set_params(grt_ctx* c, const grt_params* p)
     if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
     const int v = validate_params(c, p);
     if (v != GRT_OK) return v;
+    const grt_params& o = c->params;
+    if (c->has_grid &&
+        (p->a_bh != o.a_bh || p->density_g_cm3_per_code != o.density_g_cm3_per_code ||
+         p->temperature_k_per_code != o.temperature_k_per_code ||
+         p->four_velocity_cm_s_per_code != o.four_velocity_cm_s_per_code ||
+         p->magnetic_gauss_per_code != o.magnetic_gauss_per_code))
+        c->grid_stale = true;
+    // The adjoint may differentiate at new transfer coefficients along the
+    // taped geodesics; any other change makes the tape another problem's.
+    if (!same_except_coefficients(*p, o)) c->tape_n = 0;
     c->params = *p;
     c->sp = grt::make_step_params(*p);
     return GRT_OK;
@@ -429,6 +478,8 @@ int grt_set_grid(grt_ctx* c, int32_t nr, int32_t nth, int32_t nph, const double*
     const size_t G = (size_t)nr * nth * nph;
     if (G > (size_t)1 << 31) return fail(c, GRT_E_INVALID, "grid has more than 2^31 cells");
     free_grid(c);
+    c->grid_stale = false;
+    c->tape_n = 0;
     grt::HostGrid hg;
     grt::prepare_grid(c->params, nr, nth, nph, r, theta, phi, rho, tgas, vel, mag, hg);
     c->nr = nr;
@@ -493,9 +544,11 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
     if (max_steps <= 0) return fail(c, GRT_E_INVALID, "integration.max_steps must be positive");
     if (!(termination_percent > 0.0) || termination_percent > 1.0)
         return fail(c, GRT_E_INVALID, "integration.termination_percent must be in (0, 1]");
+    int rc = check_grid_fresh(c);
+    if (rc != GRT_OK) return rc;
     hipStream_t stream = static_cast<hipStream_t>(stream_v);
     GRT_HIP(c, hipSetDevice(c->device));
-    int rc = ensure_scratch(c, n);
+    rc = ensure_scratch(c, n);
     if (rc != GRT_OK) return rc;
 
     // Fluid-dependent features are active only with a grid, integrate_cart_ks.hpp:436-452.
@@ -664,6 +717,7 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
     st.rejected_attempts = hs[grt::kStatRejected];
     st.failed_steps = hs[grt::kStatFailed];
     st.fluid_samples = hs[grt::kStatSamples];
+    st.fluid_hits = hs[grt::kStatHits];
     st.global_steps = global_steps;
     st.terminated = n - live;
     st.launches = launches;
@@ -701,6 +755,7 @@ int grt_adjoint_device(grt_ctx* c, const double* dL_dI, uint64_t n, double* grad
     if (!dL_dI || !grad_rho || !grad_tgas || !grad_coeff)
         return fail(c, GRT_E_INVALID, "seed and gradient arrays must be non-NULL");
     if (!c->has_grid) return fail(c, GRT_E_NOGRID, "the adjoint needs a fluid grid (grt_set_grid)");
+    if (check_grid_fresh(c) != GRT_OK) return GRT_E_STATE;
     if (c->tape_n == 0 || c->tape_n != n)
         return fail(c, GRT_E_STATE, "no taped trace of this many photons: call grt_trace_taped first");
     if (c->params.scattering_enabled)
@@ -764,6 +819,8 @@ int grt_set_fluid_scalars_device(grt_ctx* c, const double* rho, const double* tg
     if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
     if (!rho || !tgas) return fail(c, GRT_E_INVALID, "rho and Tgas must be non-NULL");
     if (!c->has_grid) return fail(c, GRT_E_NOGRID, "no grid: call grt_set_grid first");
+    if (check_grid_fresh(c) != GRT_OK) return GRT_E_STATE;
+    c->tape_n = 0;   // the taped trace saw the old fluid
     GRT_HIP(c, hipSetDevice(c->device));
     GRT_HIP(c, grt::launch_set_fluid_scalars(c->d_cells, c->d_rho_code, rho, tgas, c->ncell,
                                              c->params.density_g_cm3_per_code,
@@ -896,6 +953,7 @@ namespace {
 int disk_offsets(grt_ctx* c, uint64_t* total, hipStream_t stream)
 {
     if (!c->has_grid) return fail(c, GRT_E_NOGRID, "disk mode needs the fluid grid (grt_set_grid)");
+    if (check_grid_fresh(c) != GRT_OK) return GRT_E_STATE;
     const uint64_t nc = c->ncell;
     if (!c->d_disk) {
         GRT_HIP(c, hipMalloc(&c->d_disk, 2 * (nc + 1) * sizeof(unsigned long long)));

@@ -1144,7 +1144,9 @@ GRT_HD void null_reproject(double s[8], const StepParams& P)
 // event, radiative transfer, scattering and the camera test.  `phase0` is the
 // phase the call started with; `taken` the affine step just integrated.
 // On arrival at the camera (phase becomes kArrived) the screen coordinates
-// are returned in (au, av).  Returns whether the fluid was sampled.
+// are returned in (au, av).  Returns kSampleTried when the call looked for
+// fluid (sample_scattering_fluid was called) plus kSampleHit when it found a
+// cell with valid fluid and tetrad.
 //
 // kMixed (the mixed-precision form) builds the fluid tetrad and evaluates
 // the attenuation factor exp(-dtau) in FP32; neither feeds back into the
@@ -1158,8 +1160,10 @@ GRT_HD void null_reproject(double s[8], const StepParams& P)
 // one or two lanes at a time.  On arrival the redshift factor eobs / eem is
 // returned in fr for the deferred photon.  Results are bit for bit those of
 // the immediate emission.
+constexpr uint32_t kSampleTried = 1u, kSampleHit = 2u;
+
 template <int kSep, int kMixed = 0, class Hook = NoHook, int kDefer = 0>
-GRT_HD bool post_step(Photon& ph, uint8_t phase0, double taken, const GridView& G,
+GRT_HD uint32_t post_step(Photon& ph, uint8_t phase0, double taken, const GridView& G,
                       const StepParams& P, PhotonRng& rng, double& au, double& av, double& fr,
                       Hook& hook)
 {
@@ -1253,7 +1257,7 @@ GRT_HD bool post_step(Photon& ph, uint8_t phase0, double taken, const GridView& 
             ph.term = 1;
         }
     }
-    return needs;
+    return (needs ? kSampleTried : 0u) | (sampled ? kSampleHit : 0u);
 }
 
 // The radius tests that open every functor call (integrate_cart_ks.hpp:248-263).
@@ -1339,7 +1343,12 @@ GRT_HD void init_image_ray(const CameraInit& C, double u, double v, double a, do
 
 // Work counters of one lane.
 struct LaneCounters {
-    uint32_t calls, accepted, rejected, failed, samples;
+    uint32_t calls, accepted, rejected, failed, samples, hits;
+    GRT_HD void count_sample(uint32_t code)
+    {
+        samples += code & kSampleTried;
+        hits += code >> 1;
+    }
 };
 
 // One photon resident in one lane between refills.
@@ -1378,7 +1387,8 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
             const double taken = ph.dl;
             rk4_step<Rhs>(ph.s, taken, P);
             ++n.accepted;
-            n.samples += post_step<kSep, kMixed, Hook, kDefer>(ph, L.phase0, taken, G, P, rng, au, av, fr, hook) ? 1u : 0u;
+            n.count_sample(post_step<kSep, kMixed, Hook, kDefer>(ph, L.phase0, taken, G, P, rng, au,
+                                                                  av, fr, hook));
             return true;
         }
         Rhs::eval(ph.s, P, L.k1);
@@ -1396,7 +1406,8 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
         // register allocator need not keep it alive across the post-step.
 #pragma unroll
         for (int q = 0; q < 8; ++q) L.k1[q] = 0.0;
-        n.samples += post_step<kSep, kMixed, Hook, kDefer>(ph, L.phase0, taken, G, P, rng, au, av, fr, hook) ? 1u : 0u;
+        n.count_sample(post_step<kSep, kMixed, Hook, kDefer>(ph, L.phase0, taken, G, P, rng, au, av,
+                                                              fr, hook));
         return true;
     }
     ++n.rejected;

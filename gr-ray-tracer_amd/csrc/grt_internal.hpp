@@ -36,6 +36,7 @@ enum StatSlot {
     kStatFailed,
     kStatSamples,
     kStatLastTermStep,   // max over photons of the global step on which they terminated
+    kStatHits,           // fluid samples that found a cell with valid fluid and tetrad
     kStatCount
 };
 

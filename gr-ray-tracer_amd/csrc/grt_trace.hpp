@@ -277,7 +277,7 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a_in)
                    (threadIdx.x >> 6);
     StageState st{0u, 0u, 0u, 0u};
 
-    LaneCounters n{0u, 0u, 0u, 0u, 0u};
+    LaneCounters n{0u, 0u, 0u, 0u, 0u, 0u};
     uint32_t last_term = 0;
 
     Lane L;
@@ -360,6 +360,7 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a_in)
     const unsigned long long s_rej = wave_sum(n.rejected);
     const unsigned long long s_fail = wave_sum(n.failed);
     const unsigned long long s_samp = wave_sum(n.samples);
+    const unsigned long long s_hits = wave_sum(n.hits);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         const uint32_t o = (uint32_t)__shfl_xor((int)last_term, off, 64);
@@ -371,6 +372,7 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a_in)
         atomicAdd(&a.stats[kStatRejected], s_rej);
         atomicAdd(&a.stats[kStatFailed], s_fail);
         atomicAdd(&a.stats[kStatSamples], s_samp);
+        atomicAdd(&a.stats[kStatHits], s_hits);
         atomicMax(&a.stats[kStatLastTermStep], (unsigned long long)last_term);
     }
 }

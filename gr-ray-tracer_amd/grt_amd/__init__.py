@@ -75,7 +75,8 @@ class Photons(C.Structure):
 class Stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "functor_calls", "accepted_steps", "rejected_attempts", "failed_steps",
-        "fluid_samples", "global_steps", "terminated", "launches")] + [("kernel_ms", C.c_double)]
+        "fluid_samples", "fluid_hits", "global_steps", "terminated", "launches")] + \
+        [("kernel_ms", C.c_double)]
 
 
 class Tuning(C.Structure):
@@ -177,14 +178,59 @@ def _ptr(a):
     return a.ctypes.data
 
 
-def photon_struct(ph):
+def _torch_dtypes(dt):
+    """torch dtypes accepted for a photon field of numpy dtype dt (same width;
+    ids may be signed, as torch has no full unsigned 64-bit arithmetic)."""
+    import torch
+    if dt == np.uint64:
+        return tuple(t for t in (torch.int64, getattr(torch, "uint64", None)) if t is not None)
+    return {np.float64: (torch.float64,), np.uint8: (torch.uint8,)}[dt]
+
+
+def photon_struct(ph, device_index=None):
+    """grt_photons over the 20 arrays of `ph`: all numpy arrays (host entry
+    points) or all torch tensors on the GPU `device_index` (device entry
+    points), one-dimensional, each of its field's dtype and all of the length
+    of x0.  Anything else raises GrtError before a pointer reaches the library
+    (a short, mistyped or host array would be read out of bounds by a kernel)."""
     s = Photons()
+    n = len(ph["x0"])
+    on_gpu = hasattr(ph["x0"], "data_ptr")
     for name, dt in PHOTON_FIELDS:
         a = ph[name]
-        if not hasattr(a, "data_ptr") and a.dtype != dt:
-            raise GrtError(f"photon field {name} must be {np.dtype(dt).name}")
+        if hasattr(a, "data_ptr") != on_gpu:
+            raise GrtError("photon fields must be all numpy arrays or all torch tensors "
+                           f"(field {name} is not)")
+        if on_gpu:
+            if not a.is_cuda:
+                raise GrtError(f"photon field {name} is a CPU tensor; the device entry "
+                               "points need GPU tensors (or pass numpy arrays)")
+            if device_index is not None and a.device.index != device_index:
+                raise GrtError(f"photon field {name} is on cuda:{a.device.index}, the "
+                               f"context on cuda:{device_index}")
+            if a.dtype not in _torch_dtypes(dt):
+                raise GrtError(f"photon field {name} must be {np.dtype(dt).name}, not {a.dtype}")
+            size = a.numel() if a.dim() == 1 else -1
+        else:
+            if not isinstance(a, np.ndarray) or a.dtype != dt:
+                raise GrtError(f"photon field {name} must be a numpy {np.dtype(dt).name} array")
+            size = a.shape[0] if a.ndim == 1 else -1
+        if size != n:
+            raise GrtError(f"photon field {name} must be one-dimensional of length {n}")
         setattr(s, name, _ptr(a))
     return s
+
+
+def _device_vector(t, what, n=None):
+    """Pointer of a float64 GPU tensor (contiguous; n elements when given)."""
+    import torch
+    if not hasattr(t, "data_ptr") or not t.is_cuda:
+        raise GrtError(f"{what} must be a GPU tensor")
+    if t.dtype != torch.float64:
+        raise GrtError(f"{what} must be float64")
+    if n is not None and t.numel() != n:
+        raise GrtError(f"{what} must have {n} elements")
+    return _ptr(t)
 
 
 def _stats_dict(st):
@@ -204,6 +250,11 @@ class Context:
         self.params = params
         self._keep = None
         self._grid_shape = None
+        # the HIP device grt_create took (the current one)
+        self._device_index = _current_device_index()
+        # generation of the last taped trace: the adjoint replays the tape a
+        # context holds, which a later taped trace replaces
+        self._tape_gen = 0
 
     def _check(self, rc):
         if rc != 0:
@@ -247,7 +298,7 @@ class Context:
         raw handle, default: torch's current stream)."""
         n = len(photons["x0"])
         st = Stats()
-        s = photon_struct(photons)
+        s = photon_struct(photons, self._device_index)
         if hasattr(photons["x0"], "data_ptr"):
             handle = _stream_handle(stream)
             self._check(self._L.grt_trace_device(self._h, C.byref(s), n, max_steps,
@@ -264,19 +315,28 @@ class Context:
         adjoint replays (photons: dict of GPU tensors)."""
         n = len(photons["x0"])
         st = Stats()
-        s = photon_struct(photons)
+        s = photon_struct(photons, self._device_index)
         self._check(self._L.grt_trace_taped_device(self._h, C.byref(s), n, max_steps,
                                                    termination_percent, output_interval,
                                                    slots_per_photon, _stream_handle(stream),
                                                    C.byref(st)))
         self._taped_n = n
+        self._tape_gen += 1
         return _stats_dict(st)
+
+    @property
+    def tape_generation(self):
+        """Counts the taped traces of this context; the adjoint replays the
+        one with the current generation."""
+        return self._tape_gen
 
     def adjoint(self, dL_dI, stream=None):
         """Reverse sweep of the last taped trace.  dL_dI: float64 GPU tensor,
         one entry per photon.  Returns (dL/drho, dL/dTgas) shaped like the
         grid and dL/d[absorption_coefficient, emission_coefficient]."""
         import torch
+        if not hasattr(dL_dI, "data_ptr") or not dL_dI.is_cuda:
+            raise GrtError("dL_dI must be a GPU tensor")
         dL_dI = dL_dI.to(torch.float64).contiguous()
         dev = dL_dI.device
         gr = torch.empty(self._grid_shape, dtype=torch.float64, device=dev)
@@ -296,8 +356,9 @@ class Context:
             t = tgas.to(torch.float64).contiguous()
             if tuple(r.shape) != self._grid_shape or tuple(t.shape) != self._grid_shape:
                 raise GrtError("rho and Tgas must have the grid's shape")
-            self._check(self._L.grt_set_fluid_scalars_device(self._h, r.data_ptr(), t.data_ptr(),
-                                                             _stream_handle(stream)))
+            self._check(self._L.grt_set_fluid_scalars_device(
+                self._h, _device_vector(r, "rho"), _device_vector(t, "Tgas"),
+                _stream_handle(stream)))
         else:
             r = np.ascontiguousarray(rho, dtype=np.float64)
             t = np.ascontiguousarray(tgas, dtype=np.float64)
@@ -309,10 +370,10 @@ class Context:
                    stream=None):
         """Fill `photons` (numpy dict or GPU tensor dict) with image-mode rays."""
         n = len(photons["x0"])
-        s = photon_struct(photons)
+        s = photon_struct(photons, self._device_index)
         if hasattr(photons["x0"], "data_ptr"):
-            su = screen_u.data_ptr() if screen_u is not None else None
-            sv = screen_v.data_ptr() if screen_v is not None else None
+            su = _device_vector(screen_u, "screen_u", n) if screen_u is not None else None
+            sv = _device_vector(screen_v, "screen_v", n) if screen_v is not None else None
             self._check(self._L.grt_init_image_device(self._h, C.byref(s), n, id_offset,
                                                       initial_step, su, sv,
                                                       _stream_handle(stream)))
@@ -321,6 +382,8 @@ class Context:
             if screen_u is not None:
                 screen_u = np.ascontiguousarray(screen_u, dtype=np.float64)
                 screen_v = np.ascontiguousarray(screen_v, dtype=np.float64)
+                if screen_u.shape != (n,) or screen_v.shape != (n,):
+                    raise GrtError(f"screen_u and screen_v must have {n} elements")
                 su, sv = screen_u.ctypes.data, screen_v.ctypes.data
             self._check(self._L.grt_init_image(self._h, C.byref(s), n, id_offset, initial_step,
                                                su, sv))
@@ -334,7 +397,7 @@ class Context:
     def init_disk(self, photons, initial_step, id_offset=0, stream=None):
         """Fill `photons` with the first len(photons) disk-mode packets."""
         n = len(photons["x0"])
-        s = photon_struct(photons)
+        s = photon_struct(photons, self._device_index)
         if hasattr(photons["x0"], "data_ptr"):
             self._check(self._L.grt_init_disk_device(self._h, C.byref(s), n, id_offset,
                                                      initial_step, _stream_handle(stream)))
@@ -345,7 +408,7 @@ class Context:
     def image_products(self, photons, nx, ny, spectrum_bins=128, spectrum_min_hz=1e8,
                        spectrum_max_hz=1e22):
         n = len(photons["x0"])
-        s = photon_struct(photons)
+        s = photon_struct(photons, self._device_index)
         if hasattr(photons["x0"], "data_ptr"):
             import torch
             dev = photons["x0"].device
@@ -382,6 +445,16 @@ class Context:
         for k, v in d.items():
             setattr(t, k, v)
         self._check(self._L.grt_set_tuning(self._h, C.byref(t)))
+
+
+def _current_device_index():
+    """torch's current GPU index (the device grt_create runs on), or None when
+    torch or a GPU is absent."""
+    try:
+        import torch
+        return torch.cuda.current_device() if torch.cuda.is_available() else None
+    except ImportError:
+        return None
 
 
 def _stream_handle(stream):

@@ -16,7 +16,7 @@ build_observation_products (observation.hpp:22-84), differentiably in I.
 """
 import torch
 
-from . import PHASE_ARRIVED
+from . import PHASE_ARRIVED, GrtError
 
 
 class _TransferIntensity(torch.autograd.Function):
@@ -30,6 +30,7 @@ class _TransferIntensity(torch.autograd.Function):
         ph = {k: v.clone() for k, v in photons0.items()}
         stats = context.trace_taped(ph, max_steps, slots)
         fctx.context = context
+        fctx.generation = context.tape_generation
         if out is not None:
             out["photons"] = ph
             out["stats"] = stats
@@ -37,6 +38,13 @@ class _TransferIntensity(torch.autograd.Function):
 
     @staticmethod
     def backward(fctx, grad_I):
+        # The context holds one tape: the last taped trace's, with the fluid
+        # and parameters it set.  A later forward on the same context replaced
+        # it, and replaying that one would return another problem's gradient.
+        if fctx.context.tape_generation != fctx.generation:
+            raise GrtError("transfer_intensity: another taped trace ran on this context after "
+                           "this forward pass; run backward before the next forward on the "
+                           "same context, or use one context per forward pass")
         gr, gt, gc = fctx.context.adjoint(grad_I.contiguous())
         return gr, gt, gc[0], gc[1], None, None, None, None, None
 

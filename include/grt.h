@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define GRT_ABI_VERSION 1
+#define GRT_ABI_VERSION 2
 
 enum {
     GRT_OK = 0,
@@ -89,6 +89,8 @@ typedef struct {
     uint64_t rejected_attempts;  /* rejected RK45 attempts */
     uint64_t failed_steps;       /* steps that hit the 12-attempt limit */
     uint64_t fluid_samples;      /* sample_scattering_fluid calls */
+    uint64_t fluid_hits;         /* of those, samples that found a cell with valid
+                                    fluid and tetrad (the call returned true) */
     uint64_t global_steps;       /* host-loop iterations the reference would run */
     uint64_t terminated;         /* photons with terminate set at return */
     uint64_t launches;           /* tracing-kernel launches */

@@ -1072,6 +1072,7 @@ static void step_photon(const orc_params *p, const orc_grid *g, orc_photons *ph,
     if (needs) {
         st->fluid_samples++;
         sampled = sample_fluid_cell(s, g, &fluid, e, &cell);
+        st->fluid_hits += sampled ? 1u : 0u;
     }
     orc_rng rng = {ph->id[idx], rng_draw ? rng_draw[idx] : 0u, p->rng_stream << 1, p->seed};
 
@@ -1173,19 +1174,20 @@ static void trace_impl(const orc_params *p_in, const orc_grid *g, orc_photons *p
     for (int32_t step = 0; step < max_steps; ++step) {
         st->global_steps++;
         if (live > 0) {
-            uint64_t calls = 0, acc = 0, rej = 0, fail = 0, samp = 0;
+            uint64_t calls = 0, acc = 0, rej = 0, fail = 0, samp = 0, hits = 0;
             int64_t nlive = 0;
-#pragma omp parallel for schedule(dynamic, 64) reduction(+ : calls, acc, rej, fail, samp, nlive)
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : calls, acc, rej, fail, samp, hits, nlive)
             for (int64_t i = 0; i < n; ++i) {
                 orc_stats ls;
                 memset(&ls, 0, sizeof ls);
                 step_photon(&p, &gl, ph, i, draws, &ls, tapes ? &tapes[i] : NULL);
                 calls += ls.functor_calls; acc += ls.accepted_steps;
                 rej += ls.rejected_attempts; fail += ls.failed_steps; samp += ls.fluid_samples;
+                hits += ls.fluid_hits;
                 nlive += ph->terminate[i] ? 0 : 1;
             }
             st->functor_calls += calls; st->accepted_steps += acc; st->rejected_attempts += rej;
-            st->failed_steps += fail; st->fluid_samples += samp;
+            st->failed_steps += fail; st->fluid_samples += samp; st->fluid_hits += hits;
             live = nlive;
         }
         if (output_interval <= 0 || step % output_interval == 0 || step == max_steps - 1) {

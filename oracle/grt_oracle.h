@@ -89,6 +89,7 @@ typedef struct {
     uint64_t fluid_samples;   /* sample_scattering_fluid calls */
     uint64_t global_steps;    /* host loop iterations executed */
     uint64_t terminated;
+    uint64_t fluid_hits;      /* samples that found a cell (the call returned true) */
 } orc_stats;
 
 /* Per-photon RNG cursor: Philox counter words (id_lo, id_hi, draw, stream). */

@@ -84,7 +84,7 @@ class Fluid(C.Structure):
 class Stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "functor_calls", "accepted_steps", "rejected_attempts", "failed_steps",
-        "fluid_samples", "global_steps", "terminated")]
+        "fluid_samples", "global_steps", "terminated", "fluid_hits")]
 
 
 class Rng(C.Structure):

@@ -141,7 +141,7 @@ int harness_trace(const grt_params* p, int nr, int nth, int nph, const double* r
     sp.absorb = sp.absorb && fluid;
     sp.emit = sp.emit && fluid;
     sp.scatter = sp.scatter && fluid;
-    LaneCounters cnt{0, 0, 0, 0, 0};
+    LaneCounters cnt{0, 0, 0, 0, 0, 0};
     for (int s = 0; s < 5; ++s) stats[s] = 0;
     for (uint64_t i = 0; i < n; ++i) {
         if (P->terminate[i]) continue;
@@ -192,6 +192,7 @@ int harness_trace(const grt_params* p, int nr, int nth, int nph, const double* r
     stats[2] = cnt.rejected;
     stats[3] = cnt.failed;
     stats[4] = cnt.samples;
+    stats[5] = cnt.hits;
     return 0;
 }
 

@@ -78,12 +78,12 @@ def _grid_args(fields):
 
 def trace(params, fields, photons, max_steps, force_general=False):
     args, keep = _grid_args(fields)
-    st = (C.c_uint64 * 5)()
+    st = (C.c_uint64 * 6)()
     s = grt_amd.photon_struct(photons)
     lib().harness_trace(C.byref(params), *args, int(force_general), C.byref(s),
                         len(photons["x0"]), max_steps, st)
     return dict(functor_calls=st[0], accepted_steps=st[1], rejected_attempts=st[2],
-                failed_steps=st[3], fluid_samples=st[4])
+                failed_steps=st[3], fluid_samples=st[4], fluid_hits=st[5])
 
 
 def locate(params, fields, radius, polar, az, force_general=False):

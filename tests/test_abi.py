@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), n
     assert set(names) <= set(grt_amd.EXPORTED)
-    assert L.grt_abi_version() == 1
+    assert L.grt_abi_version() == 2
 
 
 def test_library_is_in_tree_and_built_for_gfx950():

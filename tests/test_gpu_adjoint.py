@@ -167,3 +167,64 @@ def test_image_mode_gradient_is_zero(oracle, gpu):
     ctx.trace_taped(g, 10000)
     gr, gt, gc = ctx.adjoint(torch.ones(len(su), dtype=torch.float64, device=gpu))
     assert not gr.any() and not gt.any() and not gc.any()
+
+
+def test_tape_and_grid_invalidation(oracle, gpu):
+    """A taped trace is replayed only while its grid, fluid and geometry
+    stand: a new fluid or grid, or a parameter change other than the two
+    transfer coefficients, drops it.  A spin or unit-scale change after
+    set_grid leaves the cell records stale until the grid is set again."""
+    import torch
+    import grt_amd
+    fields = small_torus()
+    kw, po, ph0 = disk_setup(oracle, fields)
+    ctx = gpu_setup(kw, fields)
+    n = len(ph0["x0"])
+    seed = torch.ones(n, dtype=torch.float64, device=gpu)
+
+    def taped():
+        ctx.trace_taped(grt_amd.to_device(ph0), 4000)
+
+    taped()
+    p = grt_amd.default_params(**dict(kw, absorption_coefficient=0.5))
+    ctx.set_params(p)                       # coefficients only: the tape stands
+    ctx.adjoint(seed)
+    ctx.set_fluid_scalars(fields["rho"], fields["Tgas"])
+    with pytest.raises(grt_amd.GrtError, match="no taped trace"):
+        ctx.adjoint(seed)
+    taped()
+    ctx.set_params(grt_amd.default_params(**dict(kw, rtol=1e-7)))
+    with pytest.raises(grt_amd.GrtError, match="no taped trace"):
+        ctx.adjoint(seed)
+    ctx.set_params(grt_amd.default_params(**kw))
+    taped()
+    ctx.set_grid(fields)
+    with pytest.raises(grt_amd.GrtError, match="no taped trace"):
+        ctx.adjoint(seed)
+    # stale cell records
+    ctx.set_params(grt_amd.default_params(**dict(kw, a_bh=0.5)))
+    with pytest.raises(grt_amd.GrtError, match="call grt_set_grid again"):
+        ctx.trace(grt_amd.to_device(ph0), 10)
+    with pytest.raises(grt_amd.GrtError, match="call grt_set_grid again"):
+        ctx.disk_photon_count()
+    ctx.set_params(grt_amd.default_params(**dict(kw, density_g_cm3_per_code=2.0)))
+    ctx.set_grid(fields)
+    ctx.trace(grt_amd.to_device(ph0), 10)
+
+
+def test_autograd_two_forwards_one_backward(oracle, gpu):
+    """loss = f(x1) + f(x2) through one context: the first backward finds its
+    tape replaced and raises instead of returning the second's gradient."""
+    import torch
+    import grt_amd
+    from grt_amd.autograd import transfer_intensity
+    fields = small_torus()
+    kw, po, ph0 = disk_setup(oracle, fields)
+    ctx = gpu_setup(kw, fields)
+    g0 = grt_amd.to_device(ph0)
+    rho = torch.from_numpy(fields["rho"]).to(gpu).requires_grad_()
+    tg = torch.from_numpy(fields["Tgas"]).to(gpu)
+    I1 = transfer_intensity(ctx, g0, rho, tg, 1.0, 1.0, max_steps=4000)
+    I2 = transfer_intensity(ctx, g0, rho * 1.01, tg, 1.0, 1.0, max_steps=4000)
+    with pytest.raises(grt_amd.GrtError, match="another taped trace"):
+        (I1.sum() + I2.sum()).backward()

@@ -116,7 +116,7 @@ def test_c1_image_64(oracle, gpu, c1):
     check_states(g, o)
     check_image(ctx, oracle, g, o, 64)
     for k in ("functor_calls", "accepted_steps", "rejected_attempts", "fluid_samples",
-              "failed_steps", "global_steps", "terminated"):
+              "fluid_hits", "failed_steps", "global_steps", "terminated"):
         assert st_g[k] == st_o[k], k
 
 
@@ -134,6 +134,7 @@ def test_image_generators(oracle, gpu, c1, generator):
     check_states(g, o)
     check_image(ctx, oracle, g, o, 32)
     assert st_g["fluid_samples"] == st_o["fluid_samples"]
+    assert st_g["fluid_hits"] == st_o["fluid_hits"]
 
 
 def test_work_list_edges(oracle, gpu, c1):
@@ -212,7 +213,7 @@ def test_precision_workload_geometry_128(oracle, gpu):
               "theta_disp", "phi_disp"):
         assert np.array_equal(g[k], o[k], equal_nan=True), k
     for k in ("functor_calls", "accepted_steps", "rejected_attempts", "fluid_samples",
-              "failed_steps", "global_steps", "terminated"):
+              "fluid_hits", "failed_steps", "global_steps", "terminated"):
         assert st_g[k] == st_o[k], k
     assert np.allclose(g["I"], o["I"], rtol=1e-14, atol=0.0)
     img_g = ctx.image_products(g, 128, 128)

@@ -50,7 +50,7 @@ def assert_same(A, B, st_o, st_h, freq_tol=1e-14):
     f = np.abs(A["frequency"] - B["frequency"]) / np.maximum(np.abs(A["frequency"]), 1e-300)
     assert f.max() <= freq_tol
     for k in ("functor_calls", "accepted_steps", "rejected_attempts", "failed_steps",
-              "fluid_samples"):
+              "fluid_samples", "fluid_hits"):
         assert st_o[k] == st_h[k], k
 
 
