@@ -22,11 +22,9 @@ using grt::PhotonPtrs;
 using grt::StepParams;
 
 // Device counters of the tracing driver, followed by the per-trace
-// statistics in the same allocation (kCounterWords keeps them 8-byte
-// aligned), and their pinned host mirror.
-constexpr size_t kCounterWords = 8;
-constexpr size_t kReadbackBytes =
-    kCounterWords * sizeof(uint32_t) + grt::kStatCount * sizeof(unsigned long long);
+// statistics in the same allocation, and their pinned host mirror.
+using grt::kCounterWords;
+using grt::kReadbackBytes;
 
 struct grt_ctx {
     grt_params params{};
@@ -76,6 +74,8 @@ struct grt_ctx {
     grt::TraceConsts* d_consts = nullptr;
     grt::TraceConsts h_consts{};   // host source of the copy to d_consts
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // result of looking up this device's fixed scratch (grt_noalloc.hip)
+    hipError_t noalloc_status = hipErrorNotInitialized;
 
     // differentiable form: geodesic checkpoints of the last taped trace and
     // the replay records of the reverse sweep
@@ -407,6 +407,9 @@ int grt_create(const grt_params* p, grt_ctx** out)
         return fail(nullptr, GRT_E_NOMEM, "device allocation failed in grt_create");
     }
     c->d_stats = reinterpret_cast<unsigned long long*>(c->d_counters + kCounterWords);
+    // this device's copy of the fixed scratch of grt_trace_device_noalloc;
+    // a failure here is reported by that entry point only
+    c->noalloc_status = grt::noalloc_scratch_init(dev);
     *out = c;
     return GRT_OK;
 }
@@ -536,7 +539,7 @@ namespace {
 // geodesic checkpoints of the differentiable form (FP64 kernel).
 int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps,
                double termination_percent, int64_t output_interval, void* stream_v,
-               grt_stats* stats, int tape_slots)
+               grt_stats* stats, int tape_slots, bool fixed_scratch = false)
 {
     if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
     if (!dev || (n > 0 && !photons_complete(dev))) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
@@ -548,8 +551,29 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
     if (rc != GRT_OK) return rc;
     hipStream_t stream = static_cast<hipStream_t>(stream_v);
     GRT_HIP(c, hipSetDevice(c->device));
-    rc = ensure_scratch(c, n);
-    if (rc != GRT_OK) return rc;
+    // Scratch: the context's own buffers (grown on demand), or the fixed
+    // array of the no-allocation entry point.
+    grt::TraceScratch S{};
+    if (fixed_scratch) {
+        if (c->noalloc_status != hipSuccess)
+            return hip_fail(c, c->noalloc_status, "fixed scratch of grt_trace_device_noalloc");
+        GRT_HIP(c, grt::noalloc_scratch(c->device, &S));
+    } else {
+        rc = ensure_scratch(c, n);
+        if (rc != GRT_OK) return rc;
+        S.lists = c->d_lists;
+        S.list_cap = c->list_cap;
+        S.rng = c->d_rng;
+        S.fpend = c->d_fpend;
+        S.fratio = c->d_fratio;
+        S.counters = c->d_counters;
+        S.consts = c->d_consts;
+        S.fill_kernel = false;
+    }
+    unsigned long long* d_stats = reinterpret_cast<unsigned long long*>(S.counters + kCounterWords);
+    auto clear = [&](void* p, size_t bytes) {
+        return S.fill_kernel ? grt::launch_zero(p, bytes, stream) : hipMemsetAsync(p, 0, bytes, stream);
+    };
 
     // Fluid-dependent features are active only with a grid, integrate_cart_ks.hpp:436-452.
     StepParams P = c->sp;
@@ -560,28 +584,29 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
 
     grt_stats st{};
     // draw counters and deferred-frequency flags; counters and statistics
-    GRT_HIP(c, hipMemsetAsync(c->d_rng, 0, c->rng_cap * sizeof(uint32_t) + n, stream));
-    GRT_HIP(c, hipMemsetAsync(c->d_counters, 0, kReadbackBytes, stream));
-    uint32_t* list[2] = {c->d_lists, c->d_lists + c->list_cap};
-    GRT_HIP(c, grt::launch_build_live(dev->terminate, n, list[0], c->d_counters + 2, stream));
+    GRT_HIP(c, clear(S.rng, n * sizeof(uint32_t)));
+    GRT_HIP(c, clear(S.fpend, n));
+    GRT_HIP(c, clear(S.counters, kReadbackBytes));
+    uint32_t* list[2] = {S.lists, S.lists + S.list_cap};
+    GRT_HIP(c, grt::launch_build_live(dev->terminate, n, list[0], S.counters + 2, stream));
     // The first launch reads the live count on the device; until its
     // readback, n bounds it.
     uint32_t live = (uint32_t)n;
 
-    c->h_consts.ph = to_ptrs(*dev, c->d_rng);
-    c->h_consts.ph.fpend = c->d_fpend;
-    c->h_consts.ph.fratio = c->d_fratio;
+    c->h_consts.ph = to_ptrs(*dev, S.rng);
+    c->h_consts.ph.fpend = S.fpend;
+    c->h_consts.ph.fratio = S.fratio;
     c->h_consts.P = P;
     c->h_consts.grid = make_grid_view(c);
-    GRT_HIP(c, hipMemcpyAsync(c->d_consts, &c->h_consts, sizeof c->h_consts,
+    GRT_HIP(c, hipMemcpyAsync(S.consts, &c->h_consts, sizeof c->h_consts,
                               hipMemcpyHostToDevice, stream));
     grt::TraceLaunch L{};
-    L.consts = c->d_consts;
+    L.consts = S.consts;
     L.integrator = P.integrator;
     L.grid = make_grid_view(c);
-    L.queue = c->d_counters + 0;
-    L.n_out = c->d_counters + 1;
-    L.stats = c->d_stats;
+    L.queue = S.counters + 0;
+    L.n_out = S.counters + 1;
+    L.stats = d_stats;
     const int lds_doubles = (c->has_grid && c->separable) ? (int)c->table_doubles : 0;
     L.lds_doubles = lds_doubles;
     L.mixed = c->params.precision == GRT_PRECISION_MIXED ? 1 : 0;
@@ -605,7 +630,7 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
         L.tape = c->d_tape;
         L.tape_hdr = c->d_thdr;
         L.tape_slots = tape_slots;
-        L.tape_overflow = c->d_counters + 3;
+        L.tape_overflow = S.counters + 3;
         L.mixed = 0;
         c->tape_n = 0;
     }
@@ -649,11 +674,11 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
             L.list_in = list[cur];
             L.list_out = list[cur ^ 1];
             L.n_in = live;
-            L.n_in_dev = launches == 0 ? c->d_counters + 2 : nullptr;
+            L.n_in_dev = launches == 0 ? S.counters + 2 : nullptr;
             L.n_total = (uint32_t)n;
             L.budget = budget;
             L.step0 = (int)step0;
-            GRT_HIP(c, hipMemsetAsync(c->d_counters, 0, 2 * sizeof(uint32_t), stream));
+            GRT_HIP(c, clear(S.counters, 2 * sizeof(uint32_t)));
             const int need_blocks = (int)((live + block - 1) / block);
             const int blocks = std::max(1, std::min(persistent_blocks, need_blocks));
             GRT_HIP(c, hipEventRecord(c->ev0, stream));
@@ -666,7 +691,7 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
             // round trip.  Counters and statistics come back in one copy.
             GRT_HIP(c, grt::launch_resolve_deferred(c->h_consts.ph, n, P.seed, P.stream << 1,
                                                     stream));
-            GRT_HIP(c, hipMemcpyAsync(c->h_readback, c->d_counters, kReadbackBytes,
+            GRT_HIP(c, hipMemcpyAsync(c->h_readback, S.counters, kReadbackBytes,
                                       hipMemcpyDeviceToHost, stream));
             GRT_HIP(c, hipStreamSynchronize(stream));
             live = reinterpret_cast<const uint32_t*>(c->h_readback)[1];
@@ -685,9 +710,9 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
         if (!exact_schedule && live == 0) break;
     }
     if (tape_slots > 0) {
-        GRT_HIP(c, grt::launch_tape_finalize(c->d_thdr, n, tape_slots, c->d_counters + 3, stream));
+        GRT_HIP(c, grt::launch_tape_finalize(c->d_thdr, n, tape_slots, S.counters + 3, stream));
         uint32_t overflow = 0;
-        GRT_HIP(c, hipMemcpyAsync(&overflow, c->d_counters + 3, sizeof overflow,
+        GRT_HIP(c, hipMemcpyAsync(&overflow, S.counters + 3, sizeof overflow,
                                   hipMemcpyDeviceToHost, stream));
         GRT_HIP(c, hipStreamSynchronize(stream));
         if (overflow)
@@ -698,7 +723,7 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
         c->tape_integrator = P.integrator;
     }
     if (launches == 0) {
-        GRT_HIP(c, hipMemcpyAsync(c->h_readback, c->d_counters, kReadbackBytes,
+        GRT_HIP(c, hipMemcpyAsync(c->h_readback, S.counters, kReadbackBytes,
                                   hipMemcpyDeviceToHost, stream));
         GRT_HIP(c, hipStreamSynchronize(stream));
     }
@@ -734,6 +759,20 @@ int grt_trace_device(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max
 {
     return trace_impl(c, dev, n, max_steps, termination_percent, output_interval, stream_v, stats,
                       0);
+}
+
+int grt_trace_device_noalloc(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps,
+                             double termination_percent, int64_t output_interval, void* stream_v,
+                             grt_stats* stats)
+{
+    if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
+    if (n > GRT_NOALLOC_MAX_PHOTONS)
+        return fail(c, GRT_E_RANGE, "grt_trace_device_noalloc takes at most " +
+                                        std::to_string(GRT_NOALLOC_MAX_PHOTONS) +
+                                        " photons per call (its fixed scratch); got " +
+                                        std::to_string(n));
+    return trace_impl(c, dev, n, max_steps, termination_percent, output_interval, stream_v, stats,
+                      0, true);
 }
 
 int grt_trace_taped_device(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps,

@@ -49,6 +49,35 @@ struct TraceConsts {
     GridView grid;   // the launch's grid (the loop swaps in the LDS tables)
 };
 
+// Counters of the tracing driver: [0] work queue, [1] photons still live
+// after a launch, [2] live count before the first launch, [3] tape
+// overflow, [4] adjoint queue; then kStatCount u64 statistics (the word
+// count keeps them 8-byte aligned).  Read back together in one copy.
+constexpr size_t kCounterWords = 8;
+constexpr size_t kReadbackBytes =
+    kCounterWords * sizeof(uint32_t) + kStatCount * sizeof(unsigned long long);
+
+// The device scratch of one trace call: the context's own allocations, or
+// the fixed array of the no-allocation entry point (grt_noalloc.hip).
+struct TraceScratch {
+    uint32_t* lists;      // two work lists of list_cap entries
+    size_t list_cap;
+    uint32_t* rng;        // per-photon RNG draw counters
+    uint8_t* fpend;       // per-photon deferred-frequency flags
+    double* fratio;       // per-photon arrival redshift factors (deferred photons)
+    uint32_t* counters;   // kReadbackBytes: driver counters, then the statistics
+    TraceConsts* consts;  // launch constants
+    bool fill_kernel;     // clear with launch_zero rather than hipMemsetAsync
+};
+
+// grt_noalloc.hip.  noalloc_scratch_init looks up this device's copy of the
+// fixed scratch array (call once per device, with that device current);
+// noalloc_scratch carves it for a trace of at most GRT_NOALLOC_MAX_PHOTONS.
+hipError_t noalloc_scratch_init(int device);
+hipError_t noalloc_scratch(int device, TraceScratch* s);
+// Zero-fills [p, p + bytes) with a kernel on `stream`.
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t stream);
+
 // One wavefront's photon stage in LDS (grt_trace.hpp): the photons of the
 // next (up to 64) work-list slots, loaded with one coalesced pass over the
 // photon arrays, so a lane that needs a new photon reads it from LDS instead of

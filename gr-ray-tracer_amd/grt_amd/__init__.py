@@ -25,6 +25,7 @@ MODE_IMAGE, MODE_DISK = 0, 1
 GEN_BLACKBODY, GEN_POWER_LAW, GEN_CUSTOM = 0, 1, 2
 PHASE_DISK, PHASE_BACKWARD, PHASE_FORWARD, PHASE_ARRIVED, PHASE_REJECTED = range(5)
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+NOALLOC_MAX_PHOTONS = 1 << 20     # GRT_NOALLOC_MAX_PHOTONS of grt.h
 
 
 class GrtError(RuntimeError):
@@ -85,7 +86,8 @@ class Tuning(C.Structure):
 
 EXPORTED = [
     "grt_params_default", "grt_create", "grt_destroy", "grt_set_params", "grt_set_grid",
-    "grt_trace", "grt_trace_device", "grt_image_products", "grt_image_products_device",
+    "grt_trace", "grt_trace_device", "grt_trace_device_noalloc", "grt_image_products",
+    "grt_image_products_device",
     "grt_init_image", "grt_init_image_device", "grt_disk_photon_count", "grt_init_disk",
     "grt_init_disk_device", "grt_trace_taped_device", "grt_adjoint_device",
     "grt_set_fluid_scalars", "grt_set_fluid_scalars_device",
@@ -117,6 +119,8 @@ def load():
         "grt_trace": (C.c_int, [vp, PH, C.c_uint64, C.c_int32, C.c_double, C.c_int64, ST]),
         "grt_trace_device": (C.c_int, [vp, PH, C.c_uint64, C.c_int32, C.c_double, C.c_int64,
                                        vp, ST]),
+        "grt_trace_device_noalloc": (C.c_int, [vp, PH, C.c_uint64, C.c_int32, C.c_double,
+                                               C.c_int64, vp, ST]),
         "grt_image_products": (C.c_int, [vp, PH, C.c_uint64, C.c_int64, C.c_int64, C.c_int64,
                                          C.c_double, C.c_double, vp, vp, vp, vp]),
         "grt_image_products_device": (C.c_int, [vp, PH, C.c_uint64, C.c_int64, C.c_int64,
@@ -307,6 +311,21 @@ class Context:
         else:
             self._check(self._L.grt_trace(self._h, C.byref(s), n, max_steps,
                                           termination_percent, output_interval, C.byref(st)))
+        return _stats_dict(st)
+
+    def trace_noalloc(self, photons, max_steps, termination_percent=1.0, output_interval=0,
+                      stream=None):
+        """trace() of GPU tensors through grt_trace_device_noalloc: no device
+        allocation, at most NOALLOC_MAX_PHOTONS photons, and calls on one GPU
+        share one fixed scratch array (serialise them)."""
+        n = len(photons["x0"])
+        st = Stats()
+        s = photon_struct(photons, self._device_index)
+        if not hasattr(photons["x0"], "data_ptr"):
+            raise GrtError("trace_noalloc takes GPU tensors")
+        self._check(self._L.grt_trace_device_noalloc(self._h, C.byref(s), n, max_steps,
+                                                     termination_percent, output_interval,
+                                                     _stream_handle(stream), C.byref(st)))
         return _stats_dict(st)
 
     def trace_taped(self, photons, max_steps, slots_per_photon=64, termination_percent=1.0,

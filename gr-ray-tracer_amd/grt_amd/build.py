@@ -17,8 +17,8 @@ CSRC = os.path.join(ROOT, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 LIB_PATH = os.path.join(ROOT, "libgrt_amd.so")
 
-SOURCES = ["grt_kernels.hip", "grt_mixed.hip", "grt_adjoint.hip", "grt_disk.hip", "grt_api.cpp",
-           "grt_io.cpp"]
+SOURCES = ["grt_kernels.hip", "grt_mixed.hip", "grt_adjoint.hip", "grt_disk.hip",
+           "grt_noalloc.hip", "grt_api.cpp", "grt_io.cpp"]
 HEADERS = ["grt_device.hpp", "grt_internal.hpp", "grt_trace.hpp"]
 # The mixed-precision kernel does not reproduce the reference's rounding
 # operation for operation: it may contract a * b + c into one FMA, divide by

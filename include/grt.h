@@ -155,6 +155,27 @@ int grt_trace_device(grt_ctx *ctx, const grt_photons *dev, uint64_t n, int32_t m
                      grt_stats *stats);
 
 /*
+ * grt_trace_device without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, semantics and results as grt_trace_device, but the work
+ * lists, per-photon RNG state, counters and launch constants live in one
+ * fixed device array of the library (about 21 MiB, held by every GPU the
+ * library runs on until the process ends), so at most
+ * GRT_NOALLOC_MAX_PHOTONS photons per call (GRT_E_RANGE beyond).  The call
+ * makes no hipMalloc, hipFree or other allocating call (the grid, which
+ * grt_set_grid placed, and the caller's photon arrays are its only other
+ * device memory); it still synchronises `stream` after every launch.
+ * Every no-allocation call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (from several threads or concurrently running streams)
+ * race and must be serialised by the caller.
+ */
+#define GRT_NOALLOC_MAX_PHOTONS (1u << 20)
+int grt_trace_device_noalloc(grt_ctx *ctx, const grt_photons *dev, uint64_t n, int32_t max_steps,
+                             double termination_percent, int64_t output_interval, void *stream,
+                             grt_stats *stats);
+
+/*
  * Replaces initialize_photons_image (initialize_photons.hpp:82-170): fills n
  * image-mode rays on the camera plane of the current params with
  * id = id_offset + i, the null covariant momentum toward the hole,
