@@ -947,6 +947,19 @@ int orc_generate_photon(const orc_params *p, const orc_fluid *cell, orc_rng *rng
     return 1;
 }
 
+void orc_sample_generator(const orc_params *p, const orc_fluid *cell, int packets, int64_t n,
+                          double *nu, double *energy)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        orc_rng rng = {(uint64_t)i, 0u, 1u, p->seed};
+        double v = 0.0, e = 0.0;
+        if (!orc_generate_photon(p, cell, &rng, packets, &v, &e)) v = 0.0;
+        nu[i] = v;
+        energy[i] = e;
+    }
+}
+
 /* ------------------------------------------------------------------------- */
 /* Camera: integrate_cart_ks.hpp:96-130                                       */
 /* ------------------------------------------------------------------------- */

@@ -147,6 +147,11 @@ int orc_maybe_scatter(double state[8], double stokes[4], int enabled, double alb
 int orc_superphotons_per_cell(double density_code, double normalization);
 int orc_generate_photon(const orc_params *p, const orc_fluid *cell, orc_rng *rng, int packets,
                         double *nu, double *energy);
+/* Test helper: n draws of orc_generate_photon for one cell, draw i from the
+ * Philox stream of photon id i (stream 1, seed p->seed); nu[i] = 0 where the
+ * generator refused. */
+void orc_sample_generator(const orc_params *p, const orc_fluid *cell, int packets, int64_t n,
+                          double *nu, double *energy);
 
 /* ---- camera, integrate_cart_ks.hpp:96-130 ---- */
 int orc_camera_plane_crossing(const orc_params *p, const double state[8], double *u, double *v);

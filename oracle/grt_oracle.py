@@ -136,6 +136,7 @@ def lib():
                                             C.POINTER(Rng)]),
             "orc_superphotons_per_cell": (C.c_int, [C.c_double, C.c_double]),
             "orc_generate_photon": (C.c_int, [P, F, C.POINTER(Rng), C.c_int, _D, _D]),
+            "orc_sample_generator": (None, [P, F, C.c_int, C.c_int64, _D, _D]),
             "orc_camera_plane_crossing": (C.c_int, [P, _D, _D, _D]),
             "orc_camera_observer_energy": (C.c_double, [P, _D]),
             "orc_step_photon": (None, [P, G, PH, C.c_int64, C.POINTER(C.c_uint32), S]),
@@ -280,6 +281,16 @@ def increment_grads(fluid, nu, dl, kabs, kemit):
     t = lib().orc_optical_depth_increment_grad(C.byref(f), nu, dl, kabs, dptr(da))
     e = lib().orc_emissivity_increment_grad(C.byref(f), nu, dl, kemit, dptr(de))
     return (t, da), (e, de)
+
+
+def sample_generator(params, fluid, packets, n):
+    """n draws (frequency, packet energy) of the configured generator for one
+    fluid cell (dict of Fluid fields); draw i from photon id i's stream."""
+    f = Fluid(**fluid) if isinstance(fluid, dict) else fluid
+    nu = np.zeros(n)
+    en = np.zeros(n)
+    lib().orc_sample_generator(C.byref(params), C.byref(f), packets, n, dptr(nu), dptr(en))
+    return nu, en
 
 
 def init_image(params, su, sv, initial_step, id_offset=0):

@@ -82,3 +82,65 @@ def test_invalid_arguments():
         fishbone_moncrief(spin=1.5, mass=1.0)
     with pytest.raises(ValueError):
         fishbone_moncrief(r_in=10.0, r_pressure_max=8.0)
+    with pytest.raises(ValueError):
+        fishbone_moncrief(r_min=1.0, spin=0.9)           # inside the horizon
+    with pytest.raises(ValueError):
+        fishbone_moncrief(gamma=1.0)
+
+
+def test_equals_the_reference_generator():
+    """The reference's own generator (create_example_data.py,
+    create_fishbone_moncrief_torus) on four small grids -- mass 2, no floor,
+    gamma 5/3, an open equipotential, Schwarzschild with the default r_min --
+    stored by tests/golden/make_fm_torus_fixtures.py: the same torus cells,
+    and every value to the rounding of the two derivations (the density
+    carries the cancellation h - 1 near the torus edge)."""
+    from grt_amd.fm_torus import fishbone_moncrief
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    from make_fm_torus_fixtures import CASES, FIELDS
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "fm_torus_reference.npz"))
+    tol = dict(r=1e-15, theta=0.0, phi=0.0, rho=1e-11, Tgas=1e-11, vel=1e-14, mag=0.0)
+    for name, kw in CASES.items():
+        got = fishbone_moncrief(**kw)
+        floor = kw.get("rho_floor", 1e-6)
+        assert np.array_equal(got["rho"] == floor, ref[f"{name}/rho"] == floor), name
+        for k in FIELDS:
+            want = ref[f"{name}/{k}"]
+            assert got[k].shape == want.shape, (name, k)
+            assert np.allclose(got[k], want, rtol=tol[k], atol=0.0), (name, k)
+
+
+def test_two_derivations_of_the_keplerian_angular_momentum():
+    """-u_phi/u_t of the prograde circular orbit from its angular velocity
+    1/(r^1.5 + a) equals the closed form of Bardeen, Press & Teukolsky
+    (1972, eqs. 2.12-2.13), L/E = (r^2 - 2 a sqrt r + a^2) / (r^1.5 - 2 sqrt r + a)."""
+    from grt_amd.fm_torus import keplerian_angular_momentum
+    for a in (0.0, 0.5, 0.9, 0.94, 0.998):
+        for r in (4.0, 6.0, 10.0, 12.0, 30.0):
+            bpt = (r * r - 2 * a * np.sqrt(r) + a * a) / (r ** 1.5 - 2 * np.sqrt(r) + a)
+            assert keplerian_angular_momentum(r, a) == pytest.approx(bpt, rel=1e-13)
+
+
+@pytest.mark.parametrize("spin,mass", [(0.9, 1.0), (0.5, 1.0), (1.88, 2.0)])
+def test_kerr_schild_four_velocity_is_normalised(oracle, spin, mass):
+    """The stored (u^t, 0, 0, u^phi) taken to Cartesian Kerr-Schild components
+    by the oracle's spherical-to-Cartesian Jacobian (fluid_frame.hpp:29-49)
+    and the position map (:16-27) is a unit timelike vector of the
+    Kerr-Schild metric, g(u, u) = -1, in and outside the torus."""
+    import ctypes as C
+    from grt_amd.fm_torus import fishbone_moncrief
+    f = fishbone_moncrief(nr=20, ntheta=12, nphi=6, r_min=1.3 * mass + 1.0, r_max=80.0 * mass,
+                          mass=mass, spin=spin, r_in=6.0 * mass, r_pressure_max=12.0 * mass)
+    L = oracle.lib()
+    worst = 0.0
+    for idx in np.ndindex(f["r"].shape):
+        r, th, ph = f["r"][idx], f["theta"][idx], f["phi"][idx]
+        X = np.array([0.0, (r * np.cos(ph) - spin * np.sin(ph)) * np.sin(th),
+                      (r * np.sin(ph) + spin * np.cos(ph)) * np.sin(th), r * np.cos(th)])
+        u = np.zeros(4)
+        L.orc_spherical_to_cartesian_vector(r, th, ph, spin, oracle.dptr(f["vel"][idx].copy()),
+                                            oracle.dptr(u))
+        g = oracle.metric(X, spin, mass)
+        worst = max(worst, abs(u @ g @ u + 1.0))
+    assert worst < 1e-12
