@@ -7,7 +7,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstddef>
+#include <functional>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -101,6 +104,9 @@ struct grt_ctx {
 namespace {
 std::string g_create_error;
 }  // namespace
+
+// The output.variables selection (grt_io.cpp).
+int grt_output_selection(const char* spec, std::vector<std::string>* names);
 
 // Message slots shared with grt_io.cpp.
 void grt_set_global_error(const std::string& msg) { g_create_error = msg; }
@@ -539,7 +545,8 @@ namespace {
 // geodesic checkpoints of the differentiable form (FP64 kernel).
 int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps,
                double termination_percent, int64_t output_interval, void* stream_v,
-               grt_stats* stats, int tape_slots, bool fixed_scratch = false)
+               grt_stats* stats, int tape_slots, bool fixed_scratch = false,
+               const std::function<int(int64_t)>* on_check = nullptr, grt_timers* timers = nullptr)
 {
     if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
     if (!dev || (n > 0 && !photons_complete(dev))) return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
@@ -646,9 +653,15 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
 
     // Global-step schedule, integrate_cart_ks.hpp:468-527.  When the stop
     // test needs every photon terminated, it cannot end the loop while any
-    // photon would still change, so one epoch of max_steps is equivalent.
+    // photon would still change, so one epoch of max_steps is equivalent --
+    // unless something happens at every check step (the output archives).
     const int64_t interval = output_interval <= 0 ? 1 : output_interval;
-    const bool exact_schedule = !((double)n * termination_percent > (double)n - 1.0);
+    const bool exact_schedule =
+        on_check != nullptr || !((double)n * termination_percent > (double)n - 1.0);
+    using Clock = std::chrono::steady_clock;
+    auto seconds_since = [](Clock::time_point t0) {
+        return std::chrono::duration<double>(Clock::now() - t0).count();
+    };
     double fraction = 0.0;
     int64_t g = 0;
     int cur = 0;
@@ -665,6 +678,7 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
         }
         int64_t left = check - g + 1;
         int64_t step0 = g;
+        const Clock::time_point t_epoch = Clock::now();
         while (left > 0 && live > 0) {
             const int budget = (int)std::min<int64_t>(left, c->tune.sub_budget);
             // With every photon live the list is a permutation of all
@@ -703,9 +717,20 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
             left -= budget;
             step0 += budget;
         }
+        if (timers) timers->geodesic_integration_s += seconds_since(t_epoch);
+        // The terminated count is the kernel's own live count (read back
+        // with the statistics after each launch), so the recount is free.
+        const Clock::time_point t_count = Clock::now();
         global_steps = (uint64_t)check + 1;
         fraction = (double)(n - live) / (double)(n > 0 ? n : 1);
         g = check + 1;
+        if (timers) timers->termination_count_s += seconds_since(t_count);
+        if (on_check) {
+            const Clock::time_point t_out = Clock::now();
+            const int orc = (*on_check)(check);
+            if (timers) timers->output_s += seconds_since(t_out);
+            if (orc != GRT_OK) return orc;
+        }
         if (fraction >= termination_percent) break;
         if (!exact_schedule && live == 0) break;
     }
@@ -888,6 +913,117 @@ int grt_set_fluid_scalars(grt_ctx* c, const double* rho, const double* tgas)
     }
     (void)hipFree(d);
     return rc;
+}
+
+int grt_integrate_device(grt_ctx* c, const grt_photons* dev, uint64_t n,
+                         const grt_run_config* run, void* stream_v, grt_stats* stats,
+                         grt_timers* timers)
+{
+    if (!c) return fail(nullptr, GRT_E_INVALID, "ctx must not be NULL");
+    if (!run) return fail(c, GRT_E_INVALID, "run must not be NULL");
+    if (!dev || (n > 0 && !photons_complete(dev)))
+        return fail(c, GRT_E_INVALID, "every photon array must be non-NULL");
+    if (run->output_interval == 0) return fail(c, GRT_E_INVALID, "output.interval must be positive");
+    if (run->output_stride == 0) return fail(c, GRT_E_INVALID, "output.stride must be positive");
+    if (run->max_steps <= 0) return fail(c, GRT_E_INVALID, "integration.max_steps must be positive");
+    std::vector<std::string> sel;
+    if (grt_output_selection(run->output_variables[0] ? run->output_variables : nullptr, &sel) !=
+        GRT_OK)
+        return fail(c, GRT_E_INVALID, grt_last_error(nullptr));
+    const bool image = c->params.mode == GRT_MODE_IMAGE;
+    if (image && (run->image_nx == 0 || run->image_ny == 0 || run->spectrum_bins == 0 ||
+                  !(run->spectrum_min_hz > 0.0) || !(run->spectrum_max_hz > run->spectrum_min_hz)))
+        return fail(c, GRT_E_INVALID, "image mode needs output.image_nx, image_ny and "
+                                      "spectrum_bins > 0 and 0 < spectrum_min_hz < spectrum_max_hz");
+    hipStream_t stream = static_cast<hipStream_t>(stream_v);
+    GRT_HIP(c, hipSetDevice(c->device));
+
+    // write_output.hpp:90-103: <dir>/<name>_step_<s>_rank0.npz
+    const std::string dir = run->output_dir;
+    const std::string stem = (dir.empty() ? std::string() : dir + "/") + run->name;
+    const uint64_t stride = run->output_stride;
+    struct Field {
+        const char* name;
+        size_t member;   // offset of the pointer in grt_photons
+        size_t elem;
+    };
+    static const Field kFields[] = {
+#define GRT_FIELD(f, e) {#f, offsetof(grt_photons, f), e}
+        GRT_FIELD(id, 8), GRT_FIELD(frequency, 8), GRT_FIELD(emission_frame_energy, 8),
+        GRT_FIELD(x0, 8), GRT_FIELD(x1, 8), GRT_FIELD(x2, 8), GRT_FIELD(x3, 8),
+        GRT_FIELD(k0, 8), GRT_FIELD(k1, 8), GRT_FIELD(k2, 8), GRT_FIELD(k3, 8),
+        GRT_FIELD(I, 8), GRT_FIELD(Q, 8), GRT_FIELD(U, 8), GRT_FIELD(V, 8),
+        GRT_FIELD(dlambda, 8), GRT_FIELD(terminate, 1), GRT_FIELD(phase, 1),
+        GRT_FIELD(theta_disp, 8), GRT_FIELD(phi_disp, 8)
+#undef GRT_FIELD
+    };
+    auto member = [](const grt_photons& p, size_t off) -> void*& {
+        return *reinterpret_cast<void**>(reinterpret_cast<char*>(const_cast<grt_photons*>(&p)) + off);
+    };
+    std::vector<std::vector<unsigned char>> host(sel.size());
+    grt_timers tm{};
+    tm.last_archive_step = -1;
+    const std::function<int(int64_t)> on_check = [&](int64_t step) -> int {
+        grt_photons h{};
+        for (size_t q = 0; q < sel.size(); ++q) {
+            const Field* f = nullptr;
+            for (const Field& k : kFields)
+                if (sel[q] == k.name) f = &k;
+            if (!f) return fail(c, GRT_E_INVALID, "unknown output variable: " + sel[q]);
+            // one contiguous copy per array (a pitched copy of single
+            // elements crawls); the writer takes every stride-th element
+            host[q].resize(std::max<uint64_t>(n, 1) * f->elem);
+            const void* src = member(*dev, f->member);
+            if (n > 0)
+                GRT_HIP(c, hipMemcpyAsync(host[q].data(), src, n * f->elem,
+                                          hipMemcpyDeviceToHost, stream));
+            member(h, f->member) = host[q].data();
+        }
+        GRT_HIP(c, hipStreamSynchronize(stream));
+        const std::string path = stem + "_step_" + std::to_string(step) + "_rank0.npz";
+        if (grt_write_photons_npz(path.c_str(), &h, n, run->output_variables[0]
+                                                          ? run->output_variables : nullptr,
+                                  stride) != GRT_OK)
+            return fail(c, GRT_E_INVALID, grt_last_error(nullptr));
+        ++tm.archives;
+        tm.last_archive_step = step;
+        return GRT_OK;
+    };
+    const int rc = trace_impl(c, dev, n, run->max_steps, run->termination_percent,
+                              (int64_t)run->output_interval, stream_v, stats, 0, false, &on_check,
+                              &tm);
+    if (timers) *timers = tm;
+    if (rc != GRT_OK || !image) return rc;
+
+    // write_image_products (write_output.hpp:107-142)
+    const int64_t nx = (int64_t)run->image_nx, ny = (int64_t)run->image_ny;
+    const int64_t bins = (int64_t)run->spectrum_bins;
+    const size_t need = (size_t)(4 * nx * ny + 2 * bins + 1);
+    if (need > c->img_cap) {
+        if (c->d_img) (void)hipFree(c->d_img);
+        c->d_img = nullptr;
+        c->img_cap = 0;
+        GRT_HIP(c, hipMalloc(&c->d_img, need * sizeof(double)));
+        c->img_cap = need;
+    }
+    double* dimg = c->d_img;
+    double* dnu = dimg + 4 * nx * ny;
+    double* dsp = dnu + bins;
+    uint64_t* darr = reinterpret_cast<uint64_t*>(dsp + bins);
+    int prc = grt_image_products_device(c, dev, n, nx, ny, bins, run->spectrum_min_hz,
+                                        run->spectrum_max_hz, dimg, dnu, dsp, darr, stream_v);
+    if (prc != GRT_OK) return prc;
+    std::vector<double> img((size_t)(4 * nx * ny)), nu((size_t)bins), sp((size_t)bins);
+    uint64_t arrived = 0;
+    GRT_HIP(c, hipMemcpy(img.data(), dimg, img.size() * sizeof(double), hipMemcpyDeviceToHost));
+    GRT_HIP(c, hipMemcpy(nu.data(), dnu, nu.size() * sizeof(double), hipMemcpyDeviceToHost));
+    GRT_HIP(c, hipMemcpy(sp.data(), dsp, sp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    GRT_HIP(c, hipMemcpy(&arrived, darr, sizeof arrived, hipMemcpyDeviceToHost));
+    const std::string path = stem + "_image_products.npz";
+    if (grt_write_image_products_npz(path.c_str(), nx, ny, bins, img.data(), nu.data(), sp.data(),
+                                     arrived) != GRT_OK)
+        return fail(c, GRT_E_INVALID, grt_last_error(nullptr));
+    return GRT_OK;
 }
 
 int grt_trace(grt_ctx* c, const grt_photons* host, uint64_t n, int32_t max_steps,

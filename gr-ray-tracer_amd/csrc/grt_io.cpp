@@ -364,6 +364,20 @@ bool is_file(const std::string& p)
 
 }  // namespace
 
+// The output.variables selection of the step archives, for the driver
+// (grt_api.cpp); NULL gives the default selection.  On a bad selection the
+// message goes to grt_last_error(NULL).
+int grt_output_selection(const char* spec, std::vector<std::string>* names)
+{
+    try {
+        *names = parse_selection(spec ? spec : "id,frequency,x0,x1,x2,x3,k0,k1,k2,k3,I,Q,U,V,phase");
+        return GRT_OK;
+    } catch (const std::exception& e) {
+        grt_set_global_error(e.what());
+        return GRT_E_INVALID;
+    }
+}
+
 extern "C" {
 
 int grt_load_config(const char* path, grt_params* p, grt_run_config* run)

@@ -80,6 +80,13 @@ class Stats(C.Structure):
         [("kernel_ms", C.c_double)]
 
 
+class Timers(C.Structure):
+    """grt_timers: the reference's per-phase Timers of integrate_geodesics."""
+    _fields_ = [("geodesic_integration_s", C.c_double), ("termination_count_s", C.c_double),
+                ("output_s", C.c_double), ("archives", C.c_uint64),
+                ("last_archive_step", C.c_int64)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("sub_budget", C.c_int32)]
 
@@ -92,7 +99,7 @@ EXPORTED = [
     "grt_init_disk_device", "grt_trace_taped_device", "grt_adjoint_device",
     "grt_set_fluid_scalars", "grt_set_fluid_scalars_device",
     "grt_load_config", "grt_read_npy", "grt_load_numpy_fields", "grt_write_image_products_npz",
-    "grt_write_photons_npz",
+    "grt_write_photons_npz", "grt_integrate_device",
     "grt_get_tuning", "grt_set_tuning", "grt_last_error", "grt_abi_version",
 ]
 
@@ -312,6 +319,30 @@ class Context:
             self._check(self._L.grt_trace(self._h, C.byref(s), n, max_steps,
                                           termination_percent, output_interval, C.byref(st)))
         return _stats_dict(st)
+
+    def integrate(self, photons, run, stream=None):
+        """integrate_geodesics with its output (grt_integrate_device): trace
+        the GPU photons with the run's step limit, termination fraction and
+        output interval, writing <output_dir>/<name>_step_<s>_rank0.npz on
+        every recount step and, in image mode, <name>_image_products.npz.
+        `run` is a grt_amd.io.RunConfig or the dict grt_amd.io.load_config
+        returns.  Returns (stats, timers) as dicts."""
+        from .io import run_config_struct
+        n = len(photons["x0"])
+        if not hasattr(photons["x0"], "data_ptr"):
+            raise GrtError("integrate takes GPU tensors")
+        s = photon_struct(photons, self._device_index)
+        rc = run_config_struct(run)
+        st = Stats()
+        tm = Timers()
+        L = self._L
+        L.grt_integrate_device.restype = C.c_int
+        L.grt_integrate_device.argtypes = [C.c_void_p, C.POINTER(Photons), C.c_uint64,
+                                           C.c_void_p, C.c_void_p, C.POINTER(Stats),
+                                           C.POINTER(Timers)]
+        self._check(L.grt_integrate_device(self._h, C.byref(s), n, C.addressof(rc),
+                                           _stream_handle(stream), C.byref(st), C.byref(tm)))
+        return _stats_dict(st), {name: getattr(tm, name) for name, _ in Timers._fields_}
 
     def trace_noalloc(self, photons, max_steps, termination_percent=1.0, output_interval=0,
                       stream=None):

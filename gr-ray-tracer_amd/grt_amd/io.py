@@ -60,6 +60,22 @@ def _err():
     return GrtError(_lib().grt_last_error(None).decode())
 
 
+def run_config_struct(run):
+    """A RunConfig from a RunConfig or from the dict load_config returns
+    (missing keys keep the reference defaults of a fresh parse)."""
+    if isinstance(run, RunConfig):
+        return run
+    r = RunConfig()
+    for name, ctype in RunConfig._fields_:
+        if name not in run:
+            continue
+        v = run[name]
+        if isinstance(v, str):
+            v = v.encode()
+        setattr(r, name, v)
+    return r
+
+
 def load_config(path):
     """Parse a reference TOML config: (Params, dict of the run settings)."""
     p = Params()
@@ -124,12 +140,26 @@ def write_photons_npz(path, photons, variables=None, stride=1):
         raise _err()
 
 
-def run(config_path, device="cuda:0"):
+def print_timers(timers, steps):
+    """The reference's timer table (display.hpp:95-120) for `steps` completed
+    iterations."""
+    rows = (("Geodesic Integration", timers["geodesic_integration_s"]),
+            ("Termination Count", timers["termination_count_s"]),
+            ("Output", timers["output_s"]))
+    total = sum(t for _, t in rows)
+    print(f"Iteration: [{steps}]")
+    for label, t in rows:
+        pct = 100.0 * t / total if total > 0 else 0.0
+        print(f"{label}[% total]: {pct}% \t[time]: {t} seconds")
+    print(f"Time elapsed: {total} seconds")
+
+
+def run(config_path, device="cuda:0", verbose=False):
     """One run of the reference program (main.cpp:246-266 with
-    integrate_geodesics, integrate_cart_ks.hpp:423-535) on one GPU: returns
-    the trace statistics and the archive paths written.  Only the final
-    output step's photon archive is written (the loop's stop rule is the
-    reference's)."""
+    integrate_geodesics, integrate_cart_ks.hpp:423-535) on one GPU: config ->
+    grid -> photons -> the global loop with its step archives on every
+    recount step and, in image mode, the image products.  Returns the trace
+    statistics, the per-phase timers and the archive paths written."""
     import torch
     import grt_amd
     params, rc = load_config(config_path)
@@ -146,18 +176,18 @@ def run(config_path, device="cuda:0"):
         n = ctx.disk_photon_count()
         ph = grt_amd.empty_photons(n, device=device)
         ctx.init_disk(ph, rc["initial_step"])
-    stats = ctx.trace(ph, rc["max_steps"], rc["termination_percent"], rc["output_interval"])
+    stats, timers = ctx.integrate(ph, rc)
     torch.cuda.synchronize()
-    out = {"stats": stats, "photons": n}
-    step = max(0, stats["global_steps"] - 1)
+    if verbose:
+        print_timers(timers, stats["global_steps"])
     name = os.path.join(rc["output_dir"], rc["name"])
-    host = to_host(ph)
-    out["step_archive"] = f"{name}_step_{step}_rank0.npz"
-    write_photons_npz(out["step_archive"], host, rc["output_variables"], rc["output_stride"])
+    steps = [s for s in range(stats["global_steps"])
+             if s % rc["output_interval"] == 0 or s == rc["max_steps"] - 1]
+    out = {"stats": stats, "timers": timers, "photons": n, "photons_final": ph,
+           "step_archives": [f"{name}_step_{s}_rank0.npz" for s in steps]}
+    out["step_archive"] = out["step_archives"][-1] if steps else None
     if params.mode == MODE_IMAGE:
-        prod = ctx.image_products(ph, rc["image_nx"], rc["image_ny"], rc["spectrum_bins"],
-                                  rc["spectrum_min_hz"], rc["spectrum_max_hz"])
         out["image_products"] = f"{name}_image_products.npz"
-        write_image_products_npz(out["image_products"], prod)
-        out["arrived_count"] = prod["arrived_count"]
+        with np.load(out["image_products"], allow_pickle=False) as z:
+            out["arrived_count"] = int(z["arrived_count"][0])
     return out

@@ -312,6 +312,35 @@ typedef struct {
 } grt_run_config;
 
 int grt_load_config(const char *path, grt_params *params, grt_run_config *run);
+
+/*
+ * The reference's Timers of integrate_geodesics (display.hpp:20-133,
+ * integrate_cart_ks.hpp:466-521; one GPU, so no "Photon Exchange"):
+ * accumulated wall seconds, plus the archives written.
+ */
+typedef struct {
+    double geodesic_integration_s;  /* "Geodesic Integration": the tracing launches */
+    double termination_count_s;     /* "Termination Count": recounting the terminated fraction */
+    double output_s;                /* "Output": the step archives */
+    uint64_t archives;              /* step archives written */
+    int64_t last_archive_step;      /* global step of the last one (-1: none) */
+} grt_timers;
+
+/*
+ * Replaces integrate_geodesics (integrate_cart_ks.hpp:423-535) with its
+ * output: traces the n device photons as grt_trace_device does with
+ * run->max_steps, run->termination_percent and run->output_interval, and on
+ * every step s on which the terminated fraction is recounted
+ * (s % output_interval == 0 or s == max_steps - 1) writes the archive
+ * <output_dir>/<name>_step_<s>_rank0.npz: the output.variables selection of
+ * every output.stride-th photon as the state after step s
+ * (write_output.hpp:26-103).  In image mode it then writes
+ * <output_dir>/<name>_image_products.npz (write_output.hpp:107-142) with the
+ * run's image and spectrum settings.  stats and timers may be NULL.
+ */
+int grt_integrate_device(grt_ctx *ctx, const grt_photons *dev, uint64_t n,
+                         const grt_run_config *run, void *stream, grt_stats *stats,
+                         grt_timers *timers);
 int grt_read_npy(const char *path, double *dst, uint64_t capacity, uint64_t shape[4],
                  int32_t *ndim);
 int grt_load_numpy_fields(grt_ctx *ctx, const char *directory);

@@ -73,6 +73,81 @@ def test_image_run_writes_reference_archives(gpu, tmp_path):
     assert os.path.basename(res["step_archive"]).startswith("t_step_")
 
 
+def _oracle_after(oracle, kw, fields, ph0, step):
+    """The oracle's photons after global step `step`: a fresh trace of steps
+    0 .. step with no early stop (the RNG draw counters restart with it, as
+    in one long run)."""
+    po = oracle.make_params(**kw)
+    G = oracle.GridHolder(fields, kw["a_bh"], kw["m_bh"])
+    o = {k: v.copy() for k, v in ph0.items()}
+    oracle.trace(po, G, o, step + 1, 1.0, 1, 8)
+    return o
+
+
+@pytest.mark.parametrize("mode", ["image", "disk"])
+def test_step_archives_hold_the_state_after_each_recount_step(oracle, gpu, tmp_path, mode):
+    """integrate_geodesics' output cadence (integrate_cart_ks.hpp:492-519):
+    an archive <name>_step_<s>_rank0.npz on every step with s % interval == 0
+    or s == max_steps - 1 until the stop, each holding every stride-th
+    photon's state after step s -- compared with the oracle's photons after
+    the same step: bit for bit, frequency and (disk mode, exp of the
+    transfer) Stokes to 1e-13.  The timers cover the three phases."""
+    import torch
+    import grt_amd
+    from conftest import c1_fields, c1_params, pixel_screen
+    fields = c1_fields()
+    if mode == "image":
+        kw = c1_params()
+        su, sv = pixel_screen(24)
+        ctx = grt_amd.Context(grt_amd.default_params(**kw))
+        ctx.set_grid(fields)
+        ph0 = oracle.init_image(oracle.make_params(**kw), su, sv, 0.1)
+        interval, max_steps, thr = 5, 200, 1.0
+    else:
+        kw = c1_params(mode=1, superphoton_count_normalization=3.0, absorption_coefficient=0.05,
+                       emission_coefficient=0.3, r_term_max=150.0)
+        ctx = grt_amd.Context(grt_amd.default_params(**kw))
+        ctx.set_grid(fields)
+        ph0 = oracle.init_disk(oracle.make_params(**kw), oracle.GridHolder(fields, 0.9, 1.0), 0.1)
+        interval, max_steps, thr = 7, 60, 0.6
+    n = len(ph0["x0"])
+    stride = 3
+    run = dict(name="cad", output_dir=str(tmp_path / "out"), output_variables="all",
+               max_steps=max_steps, termination_percent=thr, output_interval=interval,
+               output_stride=stride, image_nx=24, image_ny=24, spectrum_bins=16,
+               spectrum_min_hz=1e8, spectrum_max_hz=1e22)
+    g = grt_amd.to_device(ph0)
+    st, tm = ctx.integrate(g, run)
+    torch.cuda.synchronize()
+    last = st["global_steps"] - 1
+    steps = [s for s in range(last + 1) if s % interval == 0 or s == max_steps - 1]
+    print(mode, st, tm, steps)
+    assert steps[-1] == last and tm["archives"] == len(steps) and tm["last_archive_step"] == last
+    assert min(tm["geodesic_integration_s"], tm["termination_count_s"], tm["output_s"]) >= 0
+    assert tm["geodesic_integration_s"] > 0 and tm["output_s"] > 0
+    out = tmp_path / "out"
+    names = sorted(os.listdir(out))
+    want = sorted([f"cad_step_{s}_rank0.npz" for s in steps] +
+                  (["cad_image_products.npz"] if mode == "image" else []))
+    assert names == want
+    for s in steps:
+        z = np.load(out / f"cad_step_{s}_rank0.npz")
+        o = _oracle_after(oracle, kw, fields, ph0, s)
+        assert len(z["x0"]) == 1 + (n - 1) // stride
+        for k in ("id", "x0", "x1", "x2", "x3", "k0", "k1", "k2", "k3", "dlambda", "terminate",
+                  "phase", "theta_disp", "phi_disp"):
+            assert np.array_equal(z[k], o[k][::stride], equal_nan=True), (s, k)
+        # the emission-frame energy and the frequency come from the kernel's
+        # factored tetrad inner product and the device log: last bits
+        for k in ("emission_frame_energy", "frequency", "I", "Q", "U", "V"):
+            ref = o[k][::stride]
+            assert np.allclose(z[k], ref, rtol=1e-13, atol=0.0, equal_nan=True), (s, k)
+    if mode == "image":
+        p = np.load(out / "cad_image_products.npz")
+        assert p["image_I"].shape == (24, 24)
+        assert int(p["arrived_count"][0]) == int((grt_amd.to_host(g)["phase"] == 3).sum())
+
+
 def test_disk_run(gpu, tmp_path):
     import grt_amd.io as io
     f, grid = bundle(tmp_path)
