@@ -152,8 +152,10 @@ int validate_params(grt_ctx* ctx, const grt_params* p)
         return fail(ctx, GRT_E_INVALID, "integration.integrator must be \"rk4\" or \"rk45\"");
     if (p->mode != GRT_MODE_IMAGE && p->mode != GRT_MODE_DISK)
         return fail(ctx, GRT_E_INVALID, "simulation.mode must be 'image' or 'disk'");
-    if (p->precision != GRT_PRECISION_FP64 && p->precision != GRT_PRECISION_MIXED)
-        return fail(ctx, GRT_E_INVALID, "precision must be GRT_PRECISION_FP64 or GRT_PRECISION_MIXED");
+    if (p->precision != GRT_PRECISION_FP64 && p->precision != GRT_PRECISION_MIXED &&
+        p->precision != GRT_PRECISION_FP64_FAST_RHS)
+        return fail(ctx, GRT_E_INVALID, "precision must be GRT_PRECISION_FP64, "
+                                        "GRT_PRECISION_MIXED or GRT_PRECISION_FP64_FAST_RHS");
     if (!finite(p->atol) || !finite(p->rtol) || !finite(p->min_step_scale) ||
         !finite(p->max_step_scale) || !finite(p->safety))
         return fail(ctx, GRT_E_INVALID, "integration tolerances and step scales must be finite");
@@ -616,7 +618,7 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
     L.stats = d_stats;
     const int lds_doubles = (c->has_grid && c->separable) ? (int)c->table_doubles : 0;
     L.lds_doubles = lds_doubles;
-    L.mixed = c->params.precision == GRT_PRECISION_MIXED ? 1 : 0;
+    L.mixed = c->params.precision;   // 0 FP64, 1 mixed, 2 FP64 with the fast right-hand side
     if (tape_slots > 0) {
         const size_t need = (size_t)n * (size_t)tape_slots * grt::kCkptDoubles;
         if (need > c->tape_cap) {

@@ -124,7 +124,8 @@ struct TraceLaunch {
     int budget;               // functor calls per photon in this launch
     int step0;                // global step index of the first call of this launch
     int lds_doubles;          // separable search tables staged in LDS (doubles)
-    int mixed;                // GRT_PRECISION_MIXED: run the mixed FP32/FP64 kernel
+    int mixed;                // GRT_PRECISION_*: 1 runs the mixed FP32/FP64 kernel, 2 its
+                              // FP64 control (both in grt_mixed.hip)
     // differentiable form: geodesic checkpoints (tape != nullptr selects the
     // checkpointing kernel)
     double* tape;             // [photon][tape_slots][kCkptDoubles]

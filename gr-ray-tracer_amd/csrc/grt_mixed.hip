@@ -41,6 +41,25 @@ __global__ __launch_bounds__(256, GRT_TRACE_WAVES_PER_EU) void trace_mixed(Trace
     trace_body<kIntegrator, kSep, RhsFast, 1>(a);
 }
 
+// The control of the mixed form: the same relaxed FP64 geodesic, with the
+// tetrad and the attenuation in FP64 as well (GRT_PRECISION_FP64_FAST_RHS).
+template <int kIntegrator, int kSep>
+__global__ __launch_bounds__(256, GRT_TRACE_WAVES_PER_EU) void trace_fp64_fast_rhs(TraceLaunch a)
+{
+    trace_body<kIntegrator, kSep, RhsFast, 0>(a);
+}
+
+template <int kIntegrator, int kSep>
+void launch_one(const TraceLaunch& a, int blocks, int block, size_t lds_bytes, hipStream_t stream)
+{
+    if (a.mixed == GRT_PRECISION_FP64_FAST_RHS)
+        hipLaunchKernelGGL((trace_fp64_fast_rhs<kIntegrator, kSep>), dim3(blocks), dim3(block),
+                           lds_bytes, stream, a);
+    else
+        hipLaunchKernelGGL((trace_mixed<kIntegrator, kSep>), dim3(blocks), dim3(block), lds_bytes,
+                           stream, a);
+}
+
 }  // namespace
 
 hipError_t launch_trace_mixed(const TraceLaunch& a, int blocks, int block, size_t lds_bytes,
@@ -48,15 +67,11 @@ hipError_t launch_trace_mixed(const TraceLaunch& a, int blocks, int block, size_
 {
     const bool sep = a.grid.separable || !a.grid.available;
     if (a.integrator == GRT_RK4) {
-        if (sep)
-            hipLaunchKernelGGL((trace_mixed<GRT_RK4, 1>), dim3(blocks), dim3(block), lds_bytes, stream, a);
-        else
-            hipLaunchKernelGGL((trace_mixed<GRT_RK4, 0>), dim3(blocks), dim3(block), lds_bytes, stream, a);
+        if (sep) launch_one<GRT_RK4, 1>(a, blocks, block, lds_bytes, stream);
+        else launch_one<GRT_RK4, 0>(a, blocks, block, lds_bytes, stream);
     } else {
-        if (sep)
-            hipLaunchKernelGGL((trace_mixed<GRT_RK45, 1>), dim3(blocks), dim3(block), lds_bytes, stream, a);
-        else
-            hipLaunchKernelGGL((trace_mixed<GRT_RK45, 0>), dim3(blocks), dim3(block), lds_bytes, stream, a);
+        if (sep) launch_one<GRT_RK45, 1>(a, blocks, block, lds_bytes, stream);
+        else launch_one<GRT_RK45, 0>(a, blocks, block, lds_bytes, stream);
     }
     return hipGetLastError();
 }

@@ -24,7 +24,7 @@ GRT_RK4, GRT_RK45 = 0, 1
 MODE_IMAGE, MODE_DISK = 0, 1
 GEN_BLACKBODY, GEN_POWER_LAW, GEN_CUSTOM = 0, 1, 2
 PHASE_DISK, PHASE_BACKWARD, PHASE_FORWARD, PHASE_ARRIVED, PHASE_REJECTED = range(5)
-PRECISION_FP64, PRECISION_MIXED = 0, 1
+PRECISION_FP64, PRECISION_MIXED, PRECISION_FP64_FAST_RHS = 0, 1, 2
 NOALLOC_MAX_PHOTONS = 1 << 20     # GRT_NOALLOC_MAX_PHOTONS of grt.h
 
 

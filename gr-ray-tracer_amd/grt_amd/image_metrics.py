@@ -3,6 +3,10 @@ mixed-precision form against the FP64 render.
 
 pixel_errors   per-pixel relative Stokes-I error over the pixels the FP64
                render lights (I != 0), plus the pixels lit in only one of them
+frequency_errors  per-ray relative error of the observed frequency (the
+               redshifted emission frequency) over the rays both renders bring
+               to the camera
+spectrum_l1    relative L1 distance of two spectra
 ring_diameter  diameter of the black-hole shadow: the 4-connected region of
                unlit pixels (I == 0) that holds the image centre, reported as
                the diameter of the circle of the same area, in M
@@ -24,6 +28,30 @@ def pixel_errors(I, I_ref):
         "image_l1_rel_err": float(np.abs(I - I_ref).sum() / max(np.abs(I_ref).sum(), 1e-300)),
     }
     return out
+
+
+def frequency_errors(freq, freq_ref, arrived, arrived_ref):
+    """Per-ray relative error of the observed frequency (emitted frequency
+    times the redshift factor E_obs / E_emit) over the rays that reach the
+    camera in both renders (one ray per pixel: the per-pixel error)."""
+    f = np.asarray(freq, dtype=np.float64)
+    g = np.asarray(freq_ref, dtype=np.float64)
+    both = np.asarray(arrived) & np.asarray(arrived_ref) & (g != 0.0)
+    rel = np.abs(f[both] - g[both]) / np.abs(g[both])
+    return {
+        "rays": int(both.sum()),
+        "arrived_only_one": int(np.sum(np.asarray(arrived) != np.asarray(arrived_ref))),
+        "median_rel_err": float(np.median(rel)) if rel.size else 0.0,
+        "max_rel_err": float(rel.max()) if rel.size else 0.0,
+        "frac_rel_err_gt_1e-4": float(np.mean(rel > 1e-4)) if rel.size else 0.0,
+    }
+
+
+def spectrum_l1(spec, spec_ref):
+    """sum |S - S_ref| / sum |S_ref| of two spectra on the same bins."""
+    s = np.asarray(spec, dtype=np.float64)
+    r = np.asarray(spec_ref, dtype=np.float64)
+    return float(np.abs(s - r).sum() / max(np.abs(r).sum(), 1e-300))
 
 
 def ring_diameter(I, width, height=None):

@@ -39,7 +39,14 @@ enum {                                              /* utils.hpp:135-141 PhotonP
     GRT_PHASE_DISK = 0, GRT_PHASE_IMAGE_BACKWARD = 1, GRT_PHASE_IMAGE_FORWARD = 2,
     GRT_PHASE_IMAGE_ARRIVED = 3, GRT_PHASE_REJECTED = 4
 };
-enum { GRT_PRECISION_FP64 = 0, GRT_PRECISION_MIXED = 1 };
+/* GRT_PRECISION_FP64: the reference's FP64 arithmetic, bit for bit the
+ * oracle's.  GRT_PRECISION_MIXED: FP64 geodesic with the reduced-algebra
+ * right-hand side and relaxed FP64 rounding, FP32 fluid tetrad and
+ * attenuation.  GRT_PRECISION_FP64_FAST_RHS: the mixed form's FP64 geodesic
+ * (reduced-algebra right-hand side, relaxed rounding) with an FP64 tetrad
+ * and attenuation -- the control that separates the algebra from the
+ * precision in the mixed form's speed-up. */
+enum { GRT_PRECISION_FP64 = 0, GRT_PRECISION_MIXED = 1, GRT_PRECISION_FP64_FAST_RHS = 2 };
 
 /*
  * Everything Geodesic_cartesian_kerr_schild captures at construction

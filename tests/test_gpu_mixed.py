@@ -88,6 +88,47 @@ def test_mixed_image_accuracy(gpu):
     assert nm <= 10.0 * nf + 1e-12
 
 
+def test_mixed_accuracy_on_the_precision_grid(gpu):
+    """The precision configuration's grid (256 x 128 x 128, a = 0.94, no
+    density floor, 17 deg, RK45 rtol 1e-7) at 256^2 rays: the mixed form and
+    its FP64 control (the same relaxed FP64 geodesic, FP64 tetrad) against
+    the FP64 render -- the median per-pixel Stokes-I error within 1e-4,
+    under 2% of the lit pixels beyond 1e-4, the per-pixel observed
+    frequency to 1e-6 in the median, the spectrum to 5% in L1 -- beside
+    the FP64 render's own sensitivity to a one-ulp shift of the screen
+    coordinates, which bounds what any form that is not bit-identical can
+    reach."""
+    import grt_amd
+    from grt_amd.fm_torus import fishbone_moncrief
+    from grt_amd.image_metrics import frequency_errors, pixel_errors, spectrum_l1
+    fields = fishbone_moncrief(nr=256, ntheta=128, nphi=128, r_min=1.5, r_max=100.0, mass=1.0,
+                               spin=0.94, r_in=6.0, r_pressure_max=12.0, rho_floor=0.0)
+    kw = c1_params(spin=0.94, theta=np.radians(17.0), atol=1e-7, rtol=1e-7,
+                   superphoton_count_normalization=1e9)
+    su, sv = pixel_screen(256)
+    out = {}
+    for name, prec, u, v in (("fp64", 0, su, sv), ("mixed", 1, su, sv), ("fast_rhs", 2, su, sv),
+                             ("ulp", 0, np.nextafter(su, np.inf), np.nextafter(sv, np.inf))):
+        out[name] = render(kw, fields, u, v, prec)
+    ph64, _, img64 = out["fp64"]
+    I64 = img64["image"][0]
+    arr64 = ph64["phase"] == grt_amd.PHASE_ARRIVED
+    floor = pixel_errors(out["ulp"][2]["image"][0], I64)
+    print("one-ulp screen shift of the FP64 render:", floor)
+    for name in ("mixed", "fast_rhs"):
+        ph, st, img = out[name]
+        e = pixel_errors(img["image"][0], I64)
+        fe = frequency_errors(ph["frequency"], ph64["frequency"],
+                              ph["phase"] == grt_amd.PHASE_ARRIVED, arr64)
+        sl = spectrum_l1(img["spectrum_I"], img64["spectrum_I"])
+        print(name, st, "\n  I:", e, "\n  frequency:", fe, "\n  spectrum L1:", sl)
+        assert e["lit_pixels"] > 40000
+        assert e["median_rel_err"] <= MEDIAN_TOL
+        assert e["frac_rel_err_gt_1e-4"] < 0.02
+        assert fe["median_rel_err"] <= 1e-6 and fe["rays"] > 60000
+        assert sl < 0.05
+
+
 def test_mixed_rk4_and_general_grid(gpu):
     """RK4 and the non-separable search in the mixed kernel: same phases as
     FP64 for almost every ray, images close."""
