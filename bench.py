@@ -57,7 +57,7 @@ def algorithmic_flops(t):
 # rocprofv3 --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, the factors a
 # plain 1 GiB device copy reads on the same counters); the profile summary
 # profiles/trace_fp64_rk45_fm_torus_256x128x128_1024px.md holds the commands.
-TRAFFIC_BYTES_DEFAULT = 423.1e6
+TRAFFIC_BYTES_DEFAULT = 423.3e6
 
 
 def workload(args):
@@ -99,6 +99,38 @@ def timed_renders(ctx, proto, steps, warmup, max_steps):
     last = batches[-1]
     del batches
     return wall, totals, last
+
+
+def batched_leg(ctx, proto, args, log, single_s):
+    """Several renders' rays traced by one call (one persistent launch drains
+    them all): the end-of-list tail -- lanes idling while the last photons
+    finish -- is paid once per call instead of once per render.  Reports the
+    per-render time beside the single-render latency of the headline."""
+    import torch
+    k = args.batch_renders
+    big = {name: torch.cat([t] * k) for name, t in proto.items()}
+    big["id"] = torch.arange(big["id"].numel(), dtype=big["id"].dtype, device=big["id"].device)
+    ms, walls, acc = [], [], 0
+    for rep in range(4):
+        ph = {name: t.clone() for name, t in big.items()}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st = ctx.trace(ph, args.max_steps)
+        torch.cuda.synchronize()
+        if rep > 0:
+            walls.append(time.perf_counter() - t0)
+            ms.append(st["kernel_ms"])
+            acc = st["accepted_steps"]
+        del ph
+    del big
+    torch.cuda.empty_cache()
+    wall = float(np.mean(walls))
+    log(f"batched: {k} renders per call, {wall * 1e3 / k:.3f} ms per render "
+        f"(single render {single_s * 1e3:.3f} ms)")
+    return {"renders_per_call": k, "ms_per_render": wall * 1e3 / k,
+            "kernel_ms_per_render": float(np.mean(ms)) / k,
+            "steps_per_s": acc / wall, "single_render_latency_ms": single_s * 1e3,
+            "gain_over_single": single_s / (wall / k)}
 
 
 def precision_leg(args, log):
@@ -480,6 +512,8 @@ def main():
     ap.add_argument("--adjoint-image", type=int, default=512)
     ap.add_argument("--adjoint-slots", type=int, default=256)
     ap.add_argument("--precision-steps", type=int, default=3)
+    ap.add_argument("--batch-renders", type=int, default=4,
+                    help="renders traced by one call in the batched leg (1: skip it)")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB and exit (calibrates "
@@ -615,6 +649,9 @@ def main():
                 max(1, totals["launches"]),
         },
     }
+
+    if args.batch_renders > 1:
+        result["batched_renders"] = batched_leg(ctx, proto, args, log, wall / args.steps)
 
     if not args.no_precision:
         del batches
