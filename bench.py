@@ -333,6 +333,12 @@ def scattering_leg(args, log):
         ph = grt_amd.empty_photons(n, device=dev)
         ctx.init_disk(ph, 0.1)
         torch.cuda.synchronize()
+        if name == "fp64" and not args.no_cpu and args.scatter_cpu_stride > 0:
+            # the device emits the oracle's packets (same Philox streams): a
+            # strided subset goes to the CPU baseline below
+            cpu_packets = {k: np.ascontiguousarray(grt_amd.to_host(
+                {q: t[::args.scatter_cpu_stride] for q, t in ph.items()})[k])
+                for k, _ in grt_amd.PHOTON_FIELDS}
         t0 = time.perf_counter()
         st = ctx.trace(ph, args.max_steps)
         torch.cuda.synchronize()
@@ -352,6 +358,23 @@ def scattering_leg(args, log):
     out["workload"] = ("disk mode, torus a=0.5 on 256x128x256, albedo 1, absorption+emission, "
                        "Philox seed 1234, RK45 rtol 1e-7")
     out["mixed_over_fp64"] = out["mixed"]["steps_per_s"] / out["fp64"]["steps_per_s"]
+    if not args.no_cpu and args.scatter_cpu_stride > 0:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import grt_oracle as O
+        threads = args.cpu_threads or host_info()["cpu_share"]
+        okw = {k: v for k, v in kw.items() if not k.endswith("_per_code") and k != "precision"}
+        G = O.GridHolder(f, 0.5, 1.0, units=dict(density_g_cm3_per_code=0.125))
+        t0 = time.perf_counter()
+        st = O.trace(O.make_params(**okw), G, cpu_packets, args.max_steps, 1.0, 0, threads)
+        dt = time.perf_counter() - t0
+        m = len(cpu_packets["x0"])
+        out["cpu_baseline"] = {
+            "value": st["accepted_steps"] / dt, "unit": "packet RK45 steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"every {args.scatter_cpu_stride}th of the packets ({m}), the same emitted "
+                      f"packets and Philox streams as the GPU, {st['accepted_steps']} steps in "
+                      f"{dt:.1f} s on {threads} threads"}
+        log(f"scattering MC cpu: {m} packets, {st['accepted_steps'] / dt:.3e} steps/s")
     return out
 
 
@@ -512,6 +535,8 @@ def main():
     ap.add_argument("--adjoint-image", type=int, default=512)
     ap.add_argument("--adjoint-slots", type=int, default=256)
     ap.add_argument("--precision-steps", type=int, default=3)
+    ap.add_argument("--scatter-cpu-stride", type=int, default=1000,
+                    help="the scattering leg's CPU baseline traces every n-th packet (0: skip)")
     ap.add_argument("--batch-renders", type=int, default=4,
                     help="renders traced by one call in the batched leg (1: skip it)")
     ap.add_argument("--quiet", action="store_true")

@@ -157,7 +157,7 @@ def test_differentiation_protocol(gpu, integrator, cells):
     I = transfer_intensity(ctx, ph0, rho, tg, ka, ke, max_steps=max_steps, slots_per_photon=256)
     L = torch.linalg.vector_norm(I - target)
     L.backward()
-    L0 = float(L)
+    L0 = float(L.detach())
     assert L0 > 0
     gr = rho.grad.cpu().numpy().ravel()
     h = 1e-6
