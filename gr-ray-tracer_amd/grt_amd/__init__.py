@@ -26,6 +26,7 @@ GEN_BLACKBODY, GEN_POWER_LAW, GEN_CUSTOM = 0, 1, 2
 PHASE_DISK, PHASE_BACKWARD, PHASE_FORWARD, PHASE_ARRIVED, PHASE_REJECTED = range(5)
 PRECISION_FP64, PRECISION_MIXED, PRECISION_FP64_FAST_RHS = 0, 1, 2
 NOALLOC_MAX_PHOTONS = 1 << 20     # GRT_NOALLOC_MAX_PHOTONS of grt.h
+ABI_VERSION = 2                   # GRT_ABI_VERSION of grt.h (the Stats layout below)
 
 
 class GrtError(RuntimeError):
@@ -153,6 +154,9 @@ def load():
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
+    if L.grt_abi_version() != ABI_VERSION:
+        raise GrtError(f"{LIB_PATH} has ABI version {L.grt_abi_version()}, this binding "
+                       f"version {ABI_VERSION}: rebuild the library")
     _lib = L
     return L
 
