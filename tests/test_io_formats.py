@@ -42,6 +42,26 @@ def test_reference_config_case(io, tmp_path):
     assert run["output_variables"] == "id,frequency"
 
 
+def test_run_config_dict_round_trip(io, tmp_path):
+    """The dict load_config returns becomes the RunConfig the integrate entry
+    point takes, field for field; a partial dict keeps the struct's zeros."""
+    path = write(tmp_path, "rt.toml",
+                 '[simulation]\nname = "rt"\n[integration]\nmax_steps = 123\n'
+                 'termination_percent = 0.5\n[output]\ninterval = 7\nstride = 3\n'
+                 'directory = "out"\nvariables = "id, x1"\n')
+    _, run = io.load_config(path)
+    rc = io.run_config_struct(run)
+    for name, _ in io.RunConfig._fields_:
+        v = getattr(rc, name)
+        v = v.decode() if isinstance(v, bytes) else v
+        want = run[name]
+        assert (bool(v) if name == "vacuum" else v) == want, name
+    assert (rc.max_steps, rc.output_interval, rc.output_stride) == (123, 7, 3)
+    assert io.run_config_struct(rc) is rc
+    part = io.run_config_struct(dict(name="x", max_steps=5))
+    assert part.name == b"x" and part.max_steps == 5 and part.output_interval == 0
+
+
 def test_defaults_and_derived_values(io, tmp_path):
     # utils.cpp:4-37, simulation_options.hpp:13-32, toml_config.hpp:222-223
     path = write(tmp_path, "d.toml", "# only a comment\n[black_hole]\nmass = 2.0\nspin = 1.2\n"
