@@ -129,6 +129,9 @@ def load():
                                        vp, ST]),
         "grt_trace_device_noalloc": (C.c_int, [vp, PH, C.c_uint64, C.c_int32, C.c_double,
                                                C.c_int64, vp, ST]),
+        # the run config pointer is a grt_amd.io.RunConfig (passed by address)
+        "grt_integrate_device": (C.c_int, [vp, PH, C.c_uint64, vp, vp, ST,
+                                           C.POINTER(Timers)]),
         "grt_image_products": (C.c_int, [vp, PH, C.c_uint64, C.c_int64, C.c_int64, C.c_int64,
                                          C.c_double, C.c_double, vp, vp, vp, vp]),
         "grt_image_products_device": (C.c_int, [vp, PH, C.c_uint64, C.c_int64, C.c_int64,
@@ -339,13 +342,9 @@ class Context:
         rc = run_config_struct(run)
         st = Stats()
         tm = Timers()
-        L = self._L
-        L.grt_integrate_device.restype = C.c_int
-        L.grt_integrate_device.argtypes = [C.c_void_p, C.POINTER(Photons), C.c_uint64,
-                                           C.c_void_p, C.c_void_p, C.POINTER(Stats),
-                                           C.POINTER(Timers)]
-        self._check(L.grt_integrate_device(self._h, C.byref(s), n, C.addressof(rc),
-                                           _stream_handle(stream), C.byref(st), C.byref(tm)))
+        self._check(self._L.grt_integrate_device(self._h, C.byref(s), n, C.addressof(rc),
+                                                 _stream_handle(stream), C.byref(st),
+                                                 C.byref(tm)))
         return _stats_dict(st), {name: getattr(tm, name) for name, _ in Timers._fields_}
 
     def trace_noalloc(self, photons, max_steps, termination_percent=1.0, output_interval=0,

@@ -105,19 +105,26 @@ def test_image_init_matches_oracle(oracle, gpu):
     assert abs(r["theta_disp"].mean()) < 0.5 and abs(r["theta_disp"].std() - 50 / np.sqrt(12)) < 0.5
 
 
-def test_c1_image_64(oracle, gpu, c1):
-    kw = c1_params()
-    su, sv = pixel_screen(64)
+@pytest.mark.parametrize("nx,norm", [(64, 1e8), (128, 1e8), (128, 1.0)])
+def test_c1_image(oracle, gpu, c1, nx, norm):
+    """The first reference configuration (64x32x32, a = 0.9, 60 deg, RK45
+    rtol 1e-8) at its 128^2 rays and at 64^2, with the emitting normalisation
+    and with the literal default 1.0 (int(rho * 1) = 0 packets below rho = 1:
+    every ray is rejected at its emission event and the image is zero)."""
+    kw = c1_params(superphoton_count_normalization=norm)
+    su, sv = pixel_screen(nx)
     ctx = gpu_ctx(kw, c1)
     po, G = oracle_parts(oracle, kw, c1)
     ph0 = oracle.init_image(po, su, sv, 0.1)
     g, o, st_g, st_o = trace_both(oracle, ctx, po, G, ph0)
     print("stats gpu", st_g, "oracle", st_o)
     check_states(g, o)
-    check_image(ctx, oracle, g, o, 64)
+    check_image(ctx, oracle, g, o, nx)
     for k in ("functor_calls", "accepted_steps", "rejected_attempts", "fluid_samples",
               "fluid_hits", "failed_steps", "global_steps", "terminated"):
         assert st_g[k] == st_o[k], k
+    if norm == 1.0:
+        assert np.all(g["phase"] == 4) and not g["I"].any()
 
 
 @pytest.mark.parametrize("generator", [1, 2])
