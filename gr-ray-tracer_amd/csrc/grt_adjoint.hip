@@ -143,7 +143,7 @@ __global__ __launch_bounds__(256, GRT_TRACE_WAVES_PER_EU) void adjoint_fp64(Adjo
             uint32_t call0;
             read_checkpoint(a.tape + ((size_t)p * a.tape_slots + k) * kCkptDoubles, L.ph, rng,
                             call0);
-            lane_reset(L);
+            lane_reset(L, P);
             const uint32_t ncalls = min(kSegmentCalls, calls_total - call0);
             h.count = 0;
             LaneCounters cnt{0u, 0u, 0u, 0u, 0u, 0u};

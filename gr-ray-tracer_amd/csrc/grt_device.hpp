@@ -30,6 +30,18 @@ constexpr double kTwoPi = 2.0 * kPi;
 
 GRT_HD bool is_finite(double x) { return __builtin_isfinite(x); }
 
+// v, passed through an empty asm statement the compiler cannot see into
+// (device code): what is computed from the result stays below the point of
+// the call, and branches on it are not threaded across that point.
+template <class V>
+GRT_HD V opaque(V v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(v));
+#endif
+    return v;
+}
+
 enum : uint8_t { kDisk = 0, kBackward = 1, kForward = 2, kArrived = 3, kRejected = 4 };
 
 // Constants captured by the step functor (integrate_cart_ks.hpp:35-61), with
@@ -364,6 +376,38 @@ GRT_HD double quarter_root(double x)
     return best;
 }
 
+// quarter_root(1.0 / err), bit for bit.  For err in (2^-500, 2^500) the
+// reciprocal and both roots take the short correctly rounded sequences and
+// the neighbours of y are its bit pattern -+ 1 (y is positive and finite
+// there), which is what nextafter returns; the candidates and their
+// residuals are those of quarter_root, in the same order.
+GRT_HD double quarter_root_rcp(double err)
+{
+#if defined(__HIP_DEVICE_COMPILE__) && !defined(GRT_RELAXED_MATH)
+    if (err > 0x1p-500 && err < 0x1p500) {
+        const double x = rcp_rn(err);
+        const double y = sqrt_rn(sqrt_rn(x));
+        const long long yb = __double_as_longlong(y);
+        double best = y, bestd = 1.0 / 0.0;
+#pragma unroll
+        for (int d = -1; d <= 1; ++d) {
+            const double c = d == 0 ? y : __longlong_as_double(yb + d);
+            const double s = c * c;
+            const double e1 = fma(c, c, -s);
+            const double q = s * s;
+            const double e2 = fma(s, s, -q);
+            const double r = fabs((q - x) + (e2 + 2.0 * s * e1));
+            if (r < bestd) {
+                bestd = r;
+                best = c;
+            }
+        }
+        return best;
+    }
+#endif
+    return quarter_root(1.0 / err);
+}
+
 // Component 4 (p_t) has a zero derivative, so every stage leaves it equal to
 // s[4]; the loops below skip it.
 #define GRT_FOR_STATE(i) _Pragma("unroll") for (int i = 0; i < 8; ++i) if (i != 4)
@@ -463,7 +507,7 @@ GRT_HD bool rk45_attempt(double s[8], double* h_io, const double k1[8], const St
         if (kMixed)
             scale = P.safety * (1.0 / sqrt(sqrt(err)));
         else
-            scale = P.safety * quarter_root(1.0 / err);
+            scale = P.safety * quarter_root_rcp(err);
         scale = fmin(P.smax, fmax(P.smin, scale));
     }
     *h_io = h * scale;
@@ -614,6 +658,15 @@ GRT_HD double pseudo_angle(double X, double Y)
 //   then pseudo(phi_k) | cos phi_k | sin phi_k | cos(thr_k) (-2: single entry)
 //   then the uint16 bucket counts of the three axes (see axis_bucket): keyed
 //   on r, on 3 - cos theta and on pseudo + 4, so that every key is positive.
+// Beyond the outer window of the last radial cell or below the inner window
+// of the first, the nearest cell is that end cell and its window test fails:
+// no cell can be found, and no search is needed.
+GRT_HD bool separable_radial_windows(const GridView& G, double radius)
+{
+    const double* R = G.tab;
+    return radius <= R[4 * G.nr - 1] && radius >= R[2 * G.nr];
+}
+
 GRT_HD bool locate_separable(const GridView& G, double radius, double ct, double X, double Y,
                              int& i_out, int& j_out, int& k_out)
 {
@@ -624,10 +677,8 @@ GRT_HD bool locate_separable(const GridView& G, double radius, double ct, double
     const uint16_t* BR = reinterpret_cast<const uint16_t*>(F + 4 * nph);
     const uint16_t* BT = BR + G.key_nb[0] + 1;
     const uint16_t* BF = BT + G.key_nb[1] + 1;
-    // radial (scattering.hpp:47-65).  Beyond the outer window of the last
-    // cell or below the inner window of the first the nearest cell is that
-    // end cell and its window test fails, so no search is needed.
-    if (!(radius <= R[4 * nr - 1]) || !(radius >= R[2 * nr])) return false;
+    // radial (scattering.hpp:47-65)
+    if (!separable_radial_windows(G, radius)) return false;
     int i;
     {
         const int lo = (G.key_nb[0] > 0 ? bucket_count_le(R, BR, G.key_shift[0], G.key_base[0],
@@ -868,30 +919,43 @@ GRT_HD bool fluid_tetrad(const KSMetric& g, const double u_in[4], T e[16], bool 
 // scattering update), cell index) and returns true; the caller's events run
 // inside that call, where the tetrad is defined on every path.
 // kSep selects the separable fast search at compile time.
+// `radius` is ks_radius of s's position; `enabled` false skips the sample
+// (returns false).
+//
+// Every way of finding no cell (disabled, no grid, a bad radius, outside the
+// separable grid's radial windows, a failed search) ends in the one flag
+// `found`, passed through opaque() before the branch into the hit path, so
+// the compiler does not thread each exit to the end of the post-step (that
+// repeats the copies of the photon state the hit path may modify on every
+// exit).  Cos theta and the azimuth plane are formed only for a radius that
+// passes the radial windows (nearly every sample of an image ray misses
+// there).
 template <int kSep, class T, class OnSample>
-GRT_HD bool sample_fluid(const double s[8], const GridView& G, const StepParams& P,
-                         bool full, OnSample&& on_sample)
+GRT_HD bool sample_fluid(const double s[8], double radius, bool enabled, const GridView& G,
+                         const StepParams& P, bool full, OnSample&& on_sample)
 {
     FluidCellCGS fl;
     T e[16];
     KSMetric g;
     uint32_t cell;
-    if (!G.available) return false;
-    const double radius = ks_radius(s[1], s[2], s[3], P.a);
-    if (!(radius > 0.0) || !is_finite(radius)) return false;
-    const double ct = fmax(-1.0, fmin(1.0, s[3] / radius));
-    const double X = radius * s[1] + P.a * s[2];
-    const double Y = radius * s[2] - P.a * s[1];
-    int i, j, k;
-    bool found;
-    if (kSep) {
-        found = locate_separable(G, radius, ct, X, Y, i, j, k);
-    } else {
-        const double polar = acos(ct);
-        double az = atan2(Y, X);
-        if (az < 0.0) az += kTwoPi;
-        found = locate_general(G, radius, polar, az, i, j, k);
+    int i = 0, j = 0, k = 0;
+    bool found = false;
+    if (enabled && G.available && radius > 0.0 && is_finite(radius) &&
+        (!kSep || separable_radial_windows(G, radius))) {
+        const double rad = opaque(radius);
+        const double ct = fmax(-1.0, fmin(1.0, s[3] / rad));
+        const double X = rad * s[1] + P.a * s[2];
+        const double Y = rad * s[2] - P.a * s[1];
+        if (kSep) {
+            found = locate_separable(G, rad, ct, X, Y, i, j, k);
+        } else {
+            const double polar = acos(ct);
+            double az = atan2(Y, X);
+            if (az < 0.0) az += kTwoPi;
+            found = locate_general(G, rad, polar, az, i, j, k);
+        }
     }
+    found = opaque(found ? 1 : 0) != 0;
     if (!found) return false;
     cell = (uint32_t)(((size_t)i * G.nth + j) * G.nph + k);
     const double* c = G.cells + (size_t)cell * kCellDoubles;
@@ -1061,10 +1125,10 @@ GRT_HD int camera_crossing(const StepParams& P, const double s[8], double& u, do
     return (fabs(u) <= P.cam_half_w && fabs(v) <= P.cam_half_h) ? 1 : -1;
 }
 
-// camera_observer_energy, integrate_cart_ks.hpp:122-130 (g_00 = 2H - 1).
-GRT_HD double observer_energy(const StepParams& P, const double s[8])
+// camera_observer_energy, integrate_cart_ks.hpp:122-130 (g_00 = 2H - 1);
+// r is ks_radius of s's position.
+GRT_HD double observer_energy(const StepParams& P, const double s[8], double r)
 {
-    const double r = ks_radius(s[1], s[2], s[3], P.a);
     const double g00 = 2.0 * ks_H(r, s[3], P.a, P.m) + (-1.0);
     if (!(g00 < 0.0)) return 0.0;
     return -s[4] * rsqrt_ieee(-g00);
@@ -1160,14 +1224,21 @@ GRT_HD void null_reproject(double s[8], const StepParams& P)
 // one or two lanes at a time.  On arrival the redshift factor eobs / eem is
 // returned in fr for the deferred photon.  Results are bit for bit those of
 // the immediate emission.
+//
+// The Kerr-Schild radius of the new position is computed once here, for the
+// fluid sample and the observer energy, and returned in r_out: no event
+// moves the photon, so it is also the radius the next call's opening tests
+// (radius_exit) read.
 constexpr uint32_t kSampleTried = 1u, kSampleHit = 2u;
 
 template <int kSep, int kMixed = 0, class Hook = NoHook, int kDefer = 0>
 GRT_HD uint32_t post_step(Photon& ph, uint8_t phase0, double taken, const GridView& G,
                       const StepParams& P, PhotonRng& rng, double& au, double& av, double& fr,
-                      Hook& hook)
+                      double& r_out, Hook& hook)
 {
     typedef typename std::conditional<kMixed != 0, float, double>::type T;
+    const double radius = ks_radius(ph.s[1], ph.s[2], ph.s[3], P.a);
+    r_out = radius;
     const bool image = (P.mode == 0);
     const bool needs = P.scatter || P.absorb || P.emit || (image && phase0 == kBackward);
     // The events run inside the fluid sample's success block (a
@@ -1179,8 +1250,8 @@ GRT_HD uint32_t post_step(Photon& ph, uint8_t phase0, double taken, const GridVi
     // emits once, so the wavefront would otherwise pay for it on most
     // iterations with one or two lanes active).
     const bool full = !(image && phase0 == kBackward) || P.generator == 2;
-    const bool sampled = needs && sample_fluid<kSep, T>(
-        ph.s, G, P, full,
+    const bool sampled = sample_fluid<kSep, T>(
+        ph.s, radius, needs, G, P, full,
         [&](const FluidCellCGS& fl, const T* e, const KSMetric& g, uint32_t cell) {
             if (image && phase0 == kBackward) {
                 double nu = 0.0, energy = 0.0;
@@ -1236,7 +1307,7 @@ GRT_HD uint32_t post_step(Photon& ph, uint8_t phase0, double taken, const GridVi
     if (image && phase0 == kForward) {
         const int crossing = camera_crossing(P, ph.s, au, av);
         if (crossing > 0) {
-            const double eobs = observer_energy(P, ph.s);
+            const double eobs = observer_energy(P, ph.s, radius);
             const double eem = ph.eem;
             if (eobs > 0.0 && eem > 0.0 && is_finite(eobs) && is_finite(eem)) {
                 if (kDefer && ph.pend)
@@ -1260,12 +1331,12 @@ GRT_HD uint32_t post_step(Photon& ph, uint8_t phase0, double taken, const GridVi
     return (needs ? kSampleTried : 0u) | (sampled ? kSampleHit : 0u);
 }
 
-// The radius tests that open every functor call (integrate_cart_ks.hpp:248-263).
-// Returns true when the photon terminated.
+// The radius tests that open every functor call (integrate_cart_ks.hpp:248-263);
+// dist is ks_radius of the photon's position.  Returns true when the photon
+// terminated.
 template <class Hook>
-GRT_HD bool radius_exit(Photon& ph, const StepParams& P, Hook& hook)
+GRT_HD bool radius_exit(Photon& ph, double dist, const StepParams& P, Hook& hook)
 {
-    const double dist = ks_radius(ph.s[1], ph.s[2], ph.s[3], P.a);
     if (dist <= P.r_min) {
         ph.stokes[0] = ph.stokes[1] = ph.stokes[2] = ph.stokes[3] = 0.0;
         hook.reset();
@@ -1351,20 +1422,31 @@ struct LaneCounters {
     }
 };
 
-// One photon resident in one lane between refills.
+// One photon resident in one lane between refills.  r_open is ks_radius of
+// the photon's position whenever no call is in progress: whoever places a
+// photon in the lane sets it (lane_reset, or the photon stage, which
+// computes it for 64 photons at once), and every call's post-step leaves the
+// radius of the position it ends at.
 struct Lane {
     Photon ph;
     double k1[8];
+    double r_open;
     int attempt;
     bool in_call;
     uint8_t phase0;
 };
 
-GRT_HD void lane_reset(Lane& L)
+GRT_HD void lane_reset_calls(Lane& L)
 {
     L.in_call = false;
     L.attempt = 0;
     L.phase0 = 0;
+}
+
+GRT_HD void lane_reset(Lane& L, const StepParams& P)
+{
+    lane_reset_calls(L);
+    L.r_open = ks_radius(L.ph.s[1], L.ph.s[2], L.ph.s[3], P.a);
 }
 
 // One iteration of a lane: at most one RK45 attempt (or one RK4 step) of the
@@ -1381,14 +1463,14 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
     Photon& ph = L.ph;
     if (!L.in_call) {
         L.phase0 = ph.phase;
-        if (radius_exit(ph, P, hook)) return true;
+        if (radius_exit(ph, L.r_open, P, hook)) return true;
         if (kIntegrator != 1) {
             // RK4, integrate_cart_ks.hpp:269-272
             const double taken = ph.dl;
             rk4_step<Rhs>(ph.s, taken, P);
             ++n.accepted;
             n.count_sample(post_step<kSep, kMixed, Hook, kDefer>(ph, L.phase0, taken, G, P, rng, au,
-                                                                  av, fr, hook));
+                                                                  av, fr, L.r_open, hook));
             return true;
         }
         Rhs::eval(ph.s, P, L.k1);
@@ -1407,7 +1489,7 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
 #pragma unroll
         for (int q = 0; q < 8; ++q) L.k1[q] = 0.0;
         n.count_sample(post_step<kSep, kMixed, Hook, kDefer>(ph, L.phase0, taken, G, P, rng, au, av,
-                                                              fr, hook));
+                                                              fr, L.r_open, hook));
         return true;
     }
     ++n.rejected;

@@ -82,7 +82,9 @@ hipError_t launch_zero(void* p, size_t bytes, hipStream_t stream);
 // next (up to 64) work-list slots, loaded with one coalesced pass over the
 // photon arrays, so a lane that needs a new photon reads it from LDS instead of
 // waiting for HBM.  Placed after the search tables, one per wavefront.
-constexpr int kStageDoubles = 15;   // s[8], stokes[4], freq, eem, dl
+// s[8], stokes[4], freq, eem, dl, and the Kerr-Schild radius of x (computed
+// once per staged photon, with all 64 lanes, for the next call's radius tests)
+constexpr int kStageDoubles = 16;
 struct WaveStage {
     double v[kStageDoubles][64];
     uint64_t id[64];

@@ -58,9 +58,11 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
 // Fills stage positions [0, count) with work-list slots base + position,
 // lane l loading position l; slots at or past n_in stay empty.  Called with
 // all 64 lanes of the wavefront active, so each photon array is read in one
-// coalesced pass.
+// coalesced pass, and the photons' Kerr-Schild radii (spin a) are computed
+// 64 at a time.
 __device__ __forceinline__ void stage_fill(WaveStage* S, const TraceLaunch& a,
-                                           const PhotonPtrs& p, uint32_t base, uint32_t count)
+                                           const PhotonPtrs& p, double spin, uint32_t base,
+                                           uint32_t count)
 {
     const uint32_t l = threadIdx.x & 63;
     const uint32_t slot = base + l;
@@ -79,11 +81,13 @@ __device__ __forceinline__ void stage_fill(WaveStage* S, const TraceLaunch& a,
     S->draw[l] = gmem(p.rng_draw)[i];
     S->phase[l] = (uint32_t)gmem(p.phase)[i] | ((uint32_t)gmem(p.fpend)[i] << 8);
     S->idx[l] = i;
+    S->v[15][l] = ks_radius(S->v[1][l], S->v[2][l], S->v[3][l], spin);
 }
 
-// Reads the photon at stage position `pos` into the lane; returns its index.
+// Reads the photon at stage position `pos` into the lane (its radius into
+// r); returns its index.
 __device__ __forceinline__ uint32_t stage_read(const WaveStage* S, uint32_t pos, Photon& ph,
-                                               PhotonRng& rng)
+                                               PhotonRng& rng, double& r)
 {
 #pragma unroll
     for (int q = 0; q < 8; ++q) ph.s[q] = S->v[q][pos];
@@ -92,6 +96,7 @@ __device__ __forceinline__ uint32_t stage_read(const WaveStage* S, uint32_t pos,
     ph.freq = S->v[12][pos];
     ph.eem = S->v[13][pos];
     ph.dl = S->v[14][pos];
+    r = S->v[15][pos];
     ph.phase = (uint8_t)S->phase[pos];
     ph.pend = (uint8_t)(S->phase[pos] >> 8);
     ph.term = 0;
@@ -106,10 +111,11 @@ __device__ __forceinline__ uint32_t stage_read(const WaveStage* S, uint32_t pos,
 // lane of the grid remain, then only as many as the wavefront needs, so the
 // end of the list is spread over the wavefronts as evenly as single claims
 // would.  Must be called with all 64 lanes active (st stays wave-uniform).
-// Returns whether this lane received a photon (then ph, rng and idx hold it).
+// Returns whether this lane received a photon (then ph, rng, idx and the
+// photon's radius r hold it).
 __device__ __forceinline__ bool stage_take(WaveStage* S, StageState& st, const TraceLaunch& a,
-                                           const PhotonPtrs& p, bool need, Photon& ph,
-                                           PhotonRng& rng, uint32_t& idx)
+                                           const PhotonPtrs& p, double spin, bool need, Photon& ph,
+                                           PhotonRng& rng, uint32_t& idx, double& r)
 {
     const uint64_t mask = __ballot(need);
     const uint32_t total = (uint32_t)__popcll(mask);
@@ -117,7 +123,7 @@ __device__ __forceinline__ bool stage_take(WaveStage* S, StageState& st, const T
     const uint32_t avail = st.end - st.next;
     bool got = false;
     if (need && rank < avail && st.next + rank < a.n_in) {
-        idx = stage_read(S, st.next + rank - st.base, ph, rng);
+        idx = stage_read(S, st.next + rank - st.base, ph, rng, r);
         got = true;
     }
     if (total > avail && !st.drained) {
@@ -127,10 +133,10 @@ __device__ __forceinline__ bool stage_take(WaveStage* S, StageState& st, const T
         uint32_t c = 0;
         if ((threadIdx.x & 63) == 0) c = atomicAdd(a.queue, want);
         const uint32_t nbase = __builtin_amdgcn_readfirstlane(c);
-        stage_fill(S, a, p, nbase, want);
+        stage_fill(S, a, p, spin, nbase, want);
         const uint32_t slot = nbase + (rank - avail);
         if (need && rank >= avail && slot < a.n_in) {
-            idx = stage_read(S, slot - nbase, ph, rng);
+            idx = stage_read(S, slot - nbase, ph, rng, r);
             got = true;
         }
         st.base = nbase;
@@ -285,10 +291,10 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a_in)
     rng.seed = P.seed;
     rng.stream = P.stream << 1;
     uint32_t idx = 0;
-    bool have = stage_take(S, st, a, PH, true, L.ph, rng, idx);
+    bool have = stage_take(S, st, a, PH, P.a, true, L.ph, rng, idx, L.r_open);
     uint64_t thdr = 0;
     if (kTape && have) thdr = a.tape_hdr[idx];
-    lane_reset(L);
+    lane_reset_calls(L);
     int calls_left = a.budget;
     int calls_done = 0;
 
@@ -342,12 +348,12 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a_in)
             }
         }
         if (__ballot(need) != 0) {
-            const bool got = stage_take(S, st, a, PH, need, L.ph, rng, idx);
+            const bool got = stage_take(S, st, a, PH, P.a, need, L.ph, rng, idx, L.r_open);
             if (need) {
                 have = got;
                 if (got) {
                     if (kTape) thdr = a.tape_hdr[idx];
-                    lane_reset(L);
+                    lane_reset_calls(L);
                     calls_left = a.budget;
                     calls_done = 0;
                 }

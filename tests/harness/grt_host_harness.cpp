@@ -156,7 +156,7 @@ int harness_trace(const grt_params* p, int nr, int nth, int nph, const double* r
         f.phase = P->phase[i];
         f.term = 0;
         f.pend = 0;
-        lane_reset(L);
+        lane_reset(L, sp);
         PhotonRng rng{P->id[i], sp.seed, sp.stream << 1, 0u};
         int calls = 0;
         double au = 0.0, av = 0.0, fr = 1.0;
