@@ -9,11 +9,12 @@
 // are RK step endpoints, so such a step sequence changes most pixels.
 // FP64 is therefore kept for the geodesic, with relaxed rounding: the
 // right-hand side is ks_rhs_fast (the same equations in about 100
-// operations instead of 248, with four reciprocals instead of 21 divisions),
+// operations instead of 248, with four reciprocals instead of 21 divisions,
+// and its reciprocals and roots one Newton step from the hardware's seeds),
 // and this file is compiled with FMA contraction, reciprocal-based division
 // (-freciprocal-math) and the hardware's approximate sqrt / rsq / rcp
-// sequences (-fapprox-func).  These change the result only at the ulp
-// level, which moves step sizes by ~1e-8 relative.
+// sequences (-fapprox-func).  These change the result at the 1e-14 level,
+// which moves step sizes by ~1e-8 relative.
 //
 // FP32 is used where nothing feeds back into the step sequence: the fluid
 // tetrad (Gram-Schmidt under the Kerr-Schild metric) and the attenuation
