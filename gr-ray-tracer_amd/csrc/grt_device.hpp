@@ -525,12 +525,12 @@ GRT_HD bool rk45_attempt(double s[8], double* h_io, const double k1[8], const St
         finite = finite && is_finite(x5[i]) && is_finite(ratio);
         err = fmax(err, ratio);
     }
-    if (!finite) {
-        *h_io = h * P.smin;
-        return false;
-    }
+    // The state update is a select, not a branch: accepted and rejected
+    // lanes leave the attempt with the state in the same registers.
     double scale;
-    if (err == 0.0) {
+    if (!finite) {
+        scale = P.smin;
+    } else if (err == 0.0) {
         scale = P.smax;
     } else {
         if (kMixed)
@@ -542,12 +542,10 @@ GRT_HD bool rk45_attempt(double s[8], double* h_io, const double k1[8], const St
         scale = fmin(P.smax, fmax(P.smin, scale));
     }
     *h_io = h * scale;
-    if (err <= 1.0) {
+    const bool accepted = finite && err <= 1.0;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) s[i] = x5[i];
-        return true;
-    }
-    return false;
+    for (int i = 0; i < 8; ++i) s[i] = accepted ? x5[i] : s[i];
+    return accepted;
 }
 
 // ---------------------------------------------------------------------------
@@ -1260,18 +1258,24 @@ GRT_HD void null_reproject(double s[8], const StepParams& P)
 // fluid sample and the observer energy, and returned in r_out: no event
 // moves the photon, so it is also the radius the next call's opening tests
 // (radius_exit) read.
+//
+// active false (a rejected RK45 attempt) makes the call change nothing but
+// r_out (the radius of the unchanged position).  Rejected lanes run the
+// post-step with the accepted ones instead of branching around it, so both
+// leave it with the photon in the same registers (no copies at a merge).
 constexpr uint32_t kSampleTried = 1u, kSampleHit = 2u;
 
 template <int kSep, int kMixed = 0, class Hook = NoHook, int kDefer = 0>
 GRT_HD uint32_t post_step(Photon& ph, uint8_t phase0, double taken, const GridView& G,
                       const StepParams& P, PhotonRng& rng, double& au, double& av, double& fr,
-                      double& r_out, Hook& hook)
+                      double& r_out, Hook& hook, bool active)
 {
     typedef typename std::conditional<kMixed != 0, float, double>::type T;
     const double radius = ks_radius(ph.s[1], ph.s[2], ph.s[3], P.a);
     r_out = radius;
     const bool image = (P.mode == 0);
-    const bool needs = P.scatter || P.absorb || P.emit || (image && phase0 == kBackward);
+    const bool needs =
+        active && (P.scatter || P.absorb || P.emit || (image && phase0 == kBackward));
     // The events run inside the fluid sample's success block (a
     // continuation), so the tetrad never flows through a merge of the
     // sample's exit paths.
@@ -1335,7 +1339,7 @@ GRT_HD uint32_t post_step(Photon& ph, uint8_t phase0, double taken, const GridVi
             }
         });
 
-    if (image && phase0 == kForward) {
+    if (active && image && phase0 == kForward) {
         const int crossing = camera_crossing(P, ph.s, au, av);
         if (crossing > 0) {
             const double eobs = observer_energy(P, ph.s, radius);
@@ -1460,7 +1464,6 @@ struct LaneCounters {
 // radius of the position it ends at.
 struct Lane {
     Photon ph;
-    double k1[8];
     double r_open;
     int attempt;
     bool in_call;
@@ -1486,6 +1489,12 @@ GRT_HD void lane_reset(Lane& L, const StepParams& P)
 // is thereby split at its retry loop, so lanes whose step is rejected retry
 // while their neighbours go on.  Returns true when the functor call
 // completed in this iteration; (au, av) are set when the photon arrived.
+//
+// k1 = f(s) is evaluated on every attempt, also on a retry, where s and so
+// k1 are unchanged: in a wavefront some lane opens a call on nearly every
+// iteration, so the evaluation costs the same issue slots either way, and
+// no k1 is carried from one iteration to the next.  A rejected attempt runs
+// the post-step too, as a no-op (post_step's `active`).
 template <int kIntegrator, int kSep, class Rhs = RhsExact, int kMixed = 0, class Hook = NoHook,
           int kDefer = 0>
 GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, PhotonRng& rng,
@@ -1501,26 +1510,24 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
             rk4_step<Rhs>(ph.s, taken, P);
             ++n.accepted;
             n.count_sample(post_step<kSep, kMixed, Hook, kDefer>(ph, L.phase0, taken, G, P, rng, au,
-                                                                  av, fr, L.r_open, hook));
+                                                                  av, fr, L.r_open, hook, true));
             return true;
         }
-        Rhs::eval(ph.s, P, L.k1);
         L.attempt = 0;
         L.in_call = true;
     }
     // One attempt of the retry loop, integrate_cart_ks.hpp:273-299.
+    double k1[8];
+    Rhs::eval(ph.s, P, k1);
     const double taken = ph.dl;
-    const bool accepted = rk45_attempt<Rhs, kMixed>(ph.s, &ph.dl, L.k1, P);
+    const bool accepted = rk45_attempt<Rhs, kMixed>(ph.s, &ph.dl, k1, P);
     ++L.attempt;
+    const uint32_t code = post_step<kSep, kMixed, Hook, kDefer>(ph, L.phase0, taken, G, P, rng, au,
+                                                                av, fr, L.r_open, hook, accepted);
     if (accepted) {
         ++n.accepted;
         L.in_call = false;
-        // k1 is recomputed when the next call opens; overwrite it so the
-        // register allocator need not keep it alive across the post-step.
-#pragma unroll
-        for (int q = 0; q < 8; ++q) L.k1[q] = 0.0;
-        n.count_sample(post_step<kSep, kMixed, Hook, kDefer>(ph, L.phase0, taken, G, P, rng, au, av,
-                                                              fr, L.r_open, hook));
+        n.count_sample(code);
         return true;
     }
     ++n.rejected;
