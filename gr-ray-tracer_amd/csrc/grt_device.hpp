@@ -167,6 +167,46 @@ GRT_HD double rcp_rn(double d)
 #endif
 }
 
+// sqrt_rn, also returning in q an approximation of 1/sqrt(x) (relative
+// error about 2^-47: twice the refined half-reciprocal of the sequence).
+GRT_HD double sqrt_rn_q(double x, double& q)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    q = h + h;
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    return fma(d, h, g);
+#else
+    q = 1.0 / sqrt(x);
+    return sqrt(x);
+#endif
+}
+
+// rcp_rn from a seed y0 of relative error below about 2^-40 instead of the
+// hardware's 2^-24 one (no v_rcp_f64, one Newton step fewer): one Newton
+// step leaves y within half an ulp and a hair of 1/d, where the final
+// exact-residual step provably returns the correctly rounded reciprocal,
+// as it does after rcp_rn's two steps.  Seeds used: the 1/sqrt(x) of
+// sqrt_rn_q for 1/sqrt_rn(x), and (1/D)^2 for 1/(D D).
+GRT_HD double rcp_rn_seeded(double d, double y0)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double e = fma(-d, y0, 1.0);
+    const double y = fma(y0, e, y0);
+    const double r = fma(-d, y, 1.0);
+    return fma(r, y, y);
+#else
+    (void)y0;
+    return 1.0 / d;
+#endif
+}
+
 // sqrt(x) and 1.0 / sqrt(x) with the IEEE results: the short sequences of
 // sqrt_rn / rcp_rn inside their exact range, the library ones outside it
 // (zero, denormal, huge or non-finite arguments).  A file compiled with
@@ -211,8 +251,9 @@ GRT_HD void ks_rhs(const double s[8], double a, double m, double out[8])
     const double az2 = az * az;
     // S > 0 and r > 0 away from the ring singularity, which photons never
     // reach (they terminate at r_term_min)
-    const double S = sqrt_rn(fma(A, A, 4.0 * az2));
-    const double r = sqrt_rn(0.5 * (A + S));
+    double qS, qr;
+    const double S = sqrt_rn_q(fma(A, A, 4.0 * az2), qS);
+    const double r = sqrt_rn_q(0.5 * (A + S), qr);
     const double tr = 2.0 * r;
     const double B = fma(r, r, a2);
     const double r2 = r * r;
@@ -222,8 +263,11 @@ GRT_HD void ks_rhs(const double s[8], double a, double m, double out[8])
     // from the correctly rounded reciprocal y = 1/d of its divisor with
     // Markstein's correction (div_rn), so the five divisors cost one
     // division each instead of one per quotient.  1/(2r) = (1/r)/2 exactly.
+    // Three of the five reciprocals start from seeds at hand (the roots'
+    // 1/sqrt and (1/D)^2) instead of a v_rcp_f64.
     const double D2 = D * D;
-    const double iS = rcp_rn(S), ir = rcp_rn(r), iB = rcp_rn(B), iD = rcp_rn(D), iD2 = rcp_rn(D2);
+    const double iS = rcp_rn_seeded(S, qS), ir = rcp_rn_seeded(r, qr), iB = rcp_rn(B);
+    const double iD = rcp_rn(D), iD2 = rcp_rn_seeded(D2, iD * iD);
     const double itr = 0.5 * ir;
     const double f = 1.0 + div_rn(A, S, iS);
     const double a2z2 = 2.0 * a2 * z;
