@@ -341,6 +341,23 @@ GRT_HD double sqrt_newton1(double x)
 #endif
 }
 
+// sqrt_newton1, also returning 1/sqrt(x) in q from the same seed and Newton
+// step (the coupled iteration refines both; about 2^-46 relative).
+GRT_HD double sqrt_newton1_q(double x, double& q)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = __builtin_amdgcn_rsq(x);
+    const double g = x * y, h = 0.5 * y;
+    const double r = fma(-h, g, 0.5);
+    const double hr = fma(h, r, h);
+    q = hr + hr;
+    return fma(g, r, g);
+#else
+    q = 1.0 / sqrt(x);
+    return sqrt(x);
+#endif
+}
+
 // Hamilton's equations of ks_rhs in fewer operations (mixed-precision form).
 // Only the contractions the equations need are formed:
 //   d_k(l.p) = (the explicit x^k dependence) + dr/dx^k * C,
@@ -361,9 +378,9 @@ GRT_HD void ks_rhs_fast(const double s[8], double a, double m, double out[8])
     const double a2 = a * a;
     const double A = (x * x + y * y + z * z) - a2;
     const double az = a * z;
-    const double S = sqrt_newton1(A * A + 4.0 * (az * az));
-    const double r = sqrt_newton1(0.5 * (A + S));
-    const double iS = rcp_newton1(S), ir = rcp_newton1(r);
+    double iS, ir;
+    const double S = sqrt_newton1_q(A * A + 4.0 * (az * az), iS);
+    const double r = sqrt_newton1_q(0.5 * (A + S), ir);
     const double r2 = r * r;
     const double iB = rcp_newton1(r2 + a2);
     const double r3 = r2 * r;
@@ -512,11 +529,11 @@ GRT_HD void rk4_step(double s[8], double h, const StepParams& P)
 // sums -- not k2..k6 -- stay live in registers.  Each product joins its sum
 // in one fma (the rounding of ks_rhs), as does the final s + h * sum.
 //
-// kMixed evaluates the controller's err^(-1/4) from the hardware's FP64
-// seeds with one Newton step each (sqrt_newton1, rcp_newton1; about 2^-46
-// relative) instead of the correctly rounded root.  It stays FP64: an FP32
-// root moves every proposal by ~1e-7
-// relative, which moves the image-mode emission and arrival points (RK step
+// kMixed evaluates the controller's err^(-1/4) = sqrt(1/sqrt(err)) from the
+// hardware's FP64 seeds with one Newton step each (sqrt_newton1_q,
+// sqrt_newton1; about 2^-45 relative) instead of the correctly rounded root.
+// It stays FP64: an FP32 root moves every proposal by ~1e-7 relative, which
+// moves the image-mode emission and arrival points (RK step
 // endpoints) across cell and pixel edges in ~13% of the lit pixels of a
 // 2048^2 torus image, against under 1% with FP64.
 template <class Rhs = RhsExact, int kMixed = 0>
@@ -577,11 +594,12 @@ GRT_HD bool rk45_attempt(double s[8], double* h_io, const double k1[8], const St
     } else if (err == 0.0) {
         scale = P.smax;
     } else {
-        if (kMixed)
-            scale = P.safety * ((err > 0x1p-900 && err < 0x1p900)
-                                    ? rcp_newton1(sqrt_newton1(sqrt_newton1(err)))
-                                    : 1.0 / sqrt(sqrt(err)));
-        else
+        if (kMixed) {
+            double q = 0.0;   // 1/sqrt(err); err^(-1/4) = sqrt(q)
+            if (err > 0x1p-900 && err < 0x1p900) sqrt_newton1_q(err, q);
+            scale = P.safety * ((err > 0x1p-900 && err < 0x1p900) ? sqrt_newton1(q)
+                                                                  : 1.0 / sqrt(sqrt(err)));
+        } else
             scale = P.safety * quarter_root_rcp(err);
         scale = fmin(P.smax, fmax(P.smin, scale));
     }
