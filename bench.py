@@ -208,6 +208,13 @@ def precision_leg(args, log):
         "pixels_per_s_mixed": n * n * m["steps"] / m["wall"],
         "accepted_steps_per_render": {k: v["totals"]["accepted_steps"] / v["steps"]
                                       for k, v in out.items()},
+        # the work of one render: on this face-on view many steps sample
+        # fluid (cell search, record gather, tetrad, transfer), work that
+        # the two forms share except for the FP32 tetrad and attenuation
+        "per_render": {k: {s: v["totals"][s] / v["steps"]
+                           for s in ("accepted_steps", "rejected_attempts", "fluid_samples",
+                                     "fluid_hits", "kernel_ms")}
+                       for k, v in out.items()},
         "stokes_I_vs_fp64": err,
         "image_l1_bar": 1e-4,
         "image_l1_meets_bar": err["image_l1_rel_err"] <= 1e-4,

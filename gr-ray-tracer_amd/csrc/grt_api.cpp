@@ -50,6 +50,7 @@ struct grt_ctx {
     double* d_tables = nullptr;   // separable search tables, then the three axes
     size_t table_doubles = 0;
     int key[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // bucket keys of the separable search
+    double win[2] = {0.0, 0.0};   // radial window bounds of the separable search
     double* d_coords = nullptr;   // r3, th3, ph3 (general path only)
     double* d_cells = nullptr;    // kCellDoubles per cell
     double* d_rho_code = nullptr; // unscaled density, for disk-mode packet counts
@@ -500,7 +501,10 @@ int grt_set_grid(grt_ctx* c, int32_t nr, int32_t nth, int32_t nph, const double*
     c->separable = hg.separable;
     for (int q = 0; q < 9; ++q) c->key[q] = hg.key[q];
     hipError_t e;
+    c->win[0] = c->win[1] = 0.0;
     if (hg.separable) {
+        c->win[0] = hg.tables[2 * (size_t)nr];
+        c->win[1] = hg.tables[4 * (size_t)nr - 1];
         std::vector<double> t = hg.tables;
         t.insert(t.end(), hg.axes.begin(), hg.axes.end());
         c->table_doubles = hg.tables.size();
@@ -536,7 +540,7 @@ GridView make_grid_view(const grt_ctx* c)
 {
     return grt::grid_view(c->nr, c->nth, c->nph, c->has_grid, c->separable, c->d_tables,
                           c->d_tables ? c->d_tables + c->table_doubles : nullptr, c->d_coords,
-                          c->d_cells, c->key);
+                          c->d_cells, c->key, c->win);
 }
 
 }  // namespace
@@ -697,6 +701,7 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
             GRT_HIP(c, clear(S.counters, 2 * sizeof(uint32_t)));
             const int need_blocks = (int)((live + block - 1) / block);
             const int blocks = std::max(1, std::min(persistent_blocks, need_blocks));
+            L.grid_lanes = (uint32_t)blocks * (uint32_t)block;
             GRT_HIP(c, hipEventRecord(c->ev0, stream));
             GRT_HIP(c, grt::launch_trace(L, blocks, block, lds_bytes, stream));
             GRT_HIP(c, hipEventRecord(c->ev1, stream));

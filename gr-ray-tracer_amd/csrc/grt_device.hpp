@@ -637,6 +637,10 @@ struct GridView {
     // bucket index of the separable search (axis 0 radial, 1 polar,
     // 2 azimuth; key_nb 0: none), see axis_bucket
     int key_shift[3], key_base[3], key_nb[3];
+    // separable path: the radial windows' outer bounds, r_0 e^-thr and
+    // r_(nr-1) e^thr (table entries 2 nr and 4 nr - 1), kept here so the test
+    // that rejects most samples reads launch constants instead of LDS
+    double win_lo, win_hi;
 };
 
 // Bucket of a positive double w for the separable search: the top bits of
@@ -754,8 +758,7 @@ GRT_HD double pseudo_angle(double X, double Y)
 // no cell can be found, and no search is needed.
 GRT_HD bool separable_radial_windows(const GridView& G, double radius)
 {
-    const double* R = G.tab;
-    return radius <= R[4 * G.nr - 1] && radius >= R[2 * G.nr];
+    return radius <= G.win_hi && radius >= G.win_lo;
 }
 
 GRT_HD bool locate_separable(const GridView& G, double radius, double ct, double X, double Y,

@@ -308,11 +308,14 @@ inline void prepare_grid(const grt_params& p, int nr, int nth, int nph, const do
 }
 
 // GridView over host (or device) copies of HostGrid's arrays.
+// win: host copies of the radial window bounds (GridView::win_lo, win_hi).
 inline GridView grid_view(int nr, int nth, int nph, bool available, bool separable,
                           const double* tables, const double* axes, const double* coords3,
-                          const double* cells, const int* key)
+                          const double* cells, const int* key, const double* win)
 {
     GridView g{};
+    g.win_lo = win[0];
+    g.win_hi = win[1];
     for (int a = 0; a < 3; ++a) {
         g.key_shift[a] = key[a];
         g.key_base[a] = key[3 + a];

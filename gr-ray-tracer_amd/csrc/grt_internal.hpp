@@ -123,6 +123,7 @@ struct TraceLaunch {
     uint32_t* list_out;       // photons still live after `budget` calls
     uint32_t* n_out;          // zeroed before launch
     unsigned long long* stats;  // kStatCount slots
+    uint32_t grid_lanes;      // threads of the launch (blocks * block)
     int budget;               // functor calls per photon in this launch
     int step0;                // global step index of the first call of this launch
     int lds_doubles;          // separable search tables staged in LDS (doubles)

@@ -127,7 +127,10 @@ __device__ __forceinline__ bool stage_take(WaveStage* S, StageState& st, const T
         got = true;
     }
     if (total > avail && !st.drained) {
-        const uint32_t lanes = gridDim.x * blockDim.x;
+        // the launch's lane count from the launch arguments: gridDim and
+        // blockDim would be reloaded from the dispatch packet here (a global
+        // load and a wait inside the loop)
+        const uint32_t lanes = a.grid_lanes;
         const uint32_t want =
             ((uint64_t)st.end + 2ull * lanes < (uint64_t)a.n_in) ? 64u : total - avail;
         uint32_t c = 0;

@@ -38,8 +38,11 @@ void prepare(const grt_params* p, int nr, int nth, int nph, const double* r, con
             std::memcpy(out.coords.data() + 2 * G, ph, G * 8);
         }
     }
+    const double win[2] = {avail && out.hg.separable ? out.hg.tables[2 * (size_t)nr] : 0.0,
+                           avail && out.hg.separable ? out.hg.tables[4 * (size_t)nr - 1] : 0.0};
     out.view = grid_view(nr, nth, nph, avail, avail && out.hg.separable, out.hg.tables.data(),
-                         out.hg.axes.data(), out.coords.data(), out.hg.cells.data(), out.hg.key);
+                         out.hg.axes.data(), out.coords.data(), out.hg.cells.data(), out.hg.key,
+                         win);
 }
 
 }  // namespace
