@@ -210,19 +210,37 @@ GRT_HD double rcp_rn_seeded(double d, double y0)
 // sqrt(x) and 1.0 / sqrt(x) with the IEEE results: the short sequences of
 // sqrt_rn / rcp_rn inside their exact range, the library ones outside it
 // (zero, denormal, huge or non-finite arguments).  A file compiled with
-// relaxed math (GRT_RELAXED_MATH) keeps the plain library calls.
+// relaxed math (GRT_RELAXED_MATH, the mixed form) takes, inside the range,
+// the hardware's v_rsq_f64 seed (relative error below 2^-24) refined by one
+// coupled Newton step instead (about 2^-46 relative).
 GRT_HD double sqrt_ieee(double x)
 {
-#if defined(__HIP_DEVICE_COMPILE__) && !defined(GRT_RELAXED_MATH)
-    if (x > 0x1p-600 && x < 0x1p600) return sqrt_rn(x);
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (x > 0x1p-600 && x < 0x1p600) {
+#if defined(GRT_RELAXED_MATH)
+        const double y = __builtin_amdgcn_rsq(x);
+        const double g = x * y, h = 0.5 * y;
+        return fma(g, fma(-h, g, 0.5), g);
+#else
+        return sqrt_rn(x);
+#endif
+    }
 #endif
     return sqrt(x);
 }
 
 GRT_HD double rsqrt_ieee(double x)
 {
-#if defined(__HIP_DEVICE_COMPILE__) && !defined(GRT_RELAXED_MATH)
-    if (x > 0x1p-600 && x < 0x1p600) return rcp_rn(sqrt_rn(x));
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (x > 0x1p-600 && x < 0x1p600) {
+#if defined(GRT_RELAXED_MATH)
+        const double y = __builtin_amdgcn_rsq(x);
+        const double g = x * y, h = 0.5 * y;
+        return 2.0 * fma(h, fma(-h, g, 0.5), h);
+#else
+        return rcp_rn(sqrt_rn(x));
+#endif
+    }
 #endif
     return 1.0 / sqrt(x);
 }
