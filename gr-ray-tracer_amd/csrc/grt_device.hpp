@@ -600,7 +600,9 @@ GRT_HD bool rk45_attempt(double s[8], double* h_io, const double k1[8], const St
         const double x4 = fma(h, X4[i], s[i]);
         x5[i] = fma(h, fma(B56, k[i], X5[i]), s[i]);
         const double sc = fma(P.rtol, fmax(fabs(x5[i]), fabs(x4)), P.atol);
-        const double ratio = fabs(x5[i] - x4) / sc;
+        // the mixed form divides with the seeded reciprocal (a zero sc, only
+        // possible with atol = 0, gives a non-finite ratio either way)
+        const double ratio = kMixed ? fabs(x5[i] - x4) * rcp_newton1(sc) : fabs(x5[i] - x4) / sc;
         finite = finite && is_finite(x5[i]) && is_finite(ratio);
         err = fmax(err, ratio);
     }
