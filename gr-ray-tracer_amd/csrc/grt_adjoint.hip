@@ -149,9 +149,9 @@ __global__ __launch_bounds__(256, GRT_TRACE_WAVES_PER_EU) void adjoint_fp64(Adjo
             LaneCounters cnt{0u, 0u, 0u, 0u, 0u, 0u};
             uint32_t done = 0;
             while (done < ncalls) {
-                double au = 0.0, av = 0.0, fr = 1.0;
+                double au = 0.0, av = 0.0;
                 if (lane_iterate<kIntegrator, kSep, RhsExact, 0, RecordHook>(L, G, P, rng, cnt,
-                                                                             au, av, fr, h)) {
+                                                                             au, av, h)) {
                     ++done;
                     if (L.ph.term) break;
                 }

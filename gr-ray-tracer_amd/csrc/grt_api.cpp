@@ -67,7 +67,6 @@ struct grt_ctx {
     // rng_cap u8 deferred-frequency flags (cleared together)
     uint32_t* d_rng = nullptr;
     uint8_t* d_fpend = nullptr;
-    double* d_fratio = nullptr;
     size_t rng_cap = 0;
     // one allocation: kCounterWords u32 counters ([0] queue, [1] n_out,
     // [2] live count, [3] tape overflow, [4] adjoint queue), then the
@@ -256,15 +255,12 @@ int ensure_scratch(grt_ctx* c, uint64_t n)
     }
     if (n > c->rng_cap) {
         if (c->d_rng) (void)hipFree(c->d_rng);
-        if (c->d_fratio) (void)hipFree(c->d_fratio);
         c->d_rng = nullptr;
         c->d_fpend = nullptr;
-        c->d_fratio = nullptr;
         c->rng_cap = 0;
         const size_t cap = std::max<uint64_t>(n, 1024);
         GRT_HIP(c, hipMalloc(&c->d_rng, cap * (sizeof(uint32_t) + sizeof(uint8_t))));
         c->d_fpend = reinterpret_cast<uint8_t*>(c->d_rng + cap);
-        GRT_HIP(c, hipMalloc(&c->d_fratio, cap * sizeof(double)));
         c->rng_cap = cap;
     }
     return GRT_OK;
@@ -427,7 +423,7 @@ int grt_destroy(grt_ctx* c)
 {
     if (!c) return GRT_OK;
     free_grid(c);
-    void* bufs[] = {c->d_lists, c->d_rng, c->d_fratio, c->d_counters, c->d_stage, c->d_img,
+    void* bufs[] = {c->d_lists, c->d_rng, c->d_counters, c->d_stage, c->d_img,
                     c->d_consts, c->d_tape, c->d_thdr, c->d_rec};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -578,7 +574,6 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
         S.list_cap = c->list_cap;
         S.rng = c->d_rng;
         S.fpend = c->d_fpend;
-        S.fratio = c->d_fratio;
         S.counters = c->d_counters;
         S.consts = c->d_consts;
         S.fill_kernel = false;
@@ -608,7 +603,6 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
 
     c->h_consts.ph = to_ptrs(*dev, S.rng);
     c->h_consts.ph.fpend = S.fpend;
-    c->h_consts.ph.fratio = S.fratio;
     c->h_consts.P = P;
     c->h_consts.grid = make_grid_view(c);
     GRT_HIP(c, hipMemcpyAsync(S.consts, &c->h_consts, sizeof c->h_consts,
@@ -711,7 +705,7 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
             // before the synchronisation, the last launch needs no second
             // round trip.  Counters and statistics come back in one copy.
             GRT_HIP(c, grt::launch_resolve_deferred(c->h_consts.ph, n, P.seed, P.stream << 1,
-                                                    stream));
+                                                    P.a, P.m, stream));
             GRT_HIP(c, hipMemcpyAsync(c->h_readback, S.counters, kReadbackBytes,
                                       hipMemcpyDeviceToHost, stream));
             GRT_HIP(c, hipStreamSynchronize(stream));

@@ -1241,9 +1241,9 @@ GRT_HD int camera_crossing(const StepParams& P, const double s[8], double& u, do
 
 // camera_observer_energy, integrate_cart_ks.hpp:122-130 (g_00 = 2H - 1);
 // r is ks_radius of s's position.
-GRT_HD double observer_energy(const StepParams& P, const double s[8], double r)
+GRT_HD double observer_energy(double a, double m, const double s[8], double r)
 {
-    const double g00 = 2.0 * ks_H(r, s[3], P.a, P.m) + (-1.0);
+    const double g00 = 2.0 * ks_H(r, s[3], a, m) + (-1.0);
     if (!(g00 < 0.0)) return 0.0;
     return -s[4] * rsqrt_ieee(-g00);
 }
@@ -1280,9 +1280,28 @@ GRT_HD void resolve_frequency(Photon& ph, const PhotonRng& rng)
     ph.pend = 0;
 }
 
+// The camera arrival of a photon whose frequency is deferred, completed after
+// the trace from its stored state (integrate_cart_ks.hpp:122-130, 392-400):
+// the redshift factor eobs / eem that the immediate form multiplies into the
+// frequency at arrival (returns true), or, where eobs or eem is not a
+// positive finite number, false with fr = 1: the caller zeroes the Stokes
+// vector.  It reads s[1..4] (position and p_t) and eem, which are those the
+// arrival test saw, so the values are the immediate form's bit for bit.
+GRT_HD bool deferred_arrival_factor(double a, double m, const Photon& ph, double& fr)
+{
+    const double r = ks_radius(ph.s[1], ph.s[2], ph.s[3], a);
+    const double eobs = observer_energy(a, m, ph.s, r);
+    const double eem = ph.eem;
+    if (eobs > 0.0 && eem > 0.0 && is_finite(eobs) && is_finite(eem)) {
+        fr = eobs / eem;
+        return true;
+    }
+    fr = 1.0;
+    return false;
+}
+
 // After the trace: the deferred frequency, times the arrival redshift fr
-// (ph.freq *= eobs / eem in the immediate form; fr = 1 where arrival left
-// the frequency unchanged) for a photon that arrived.
+// (deferred_arrival_factor) for a photon that arrived.
 GRT_HD void finish_deferred(Photon& ph, const PhotonRng& rng, double fr)
 {
     resolve_frequency(ph, rng);
@@ -1335,9 +1354,11 @@ GRT_HD void null_reproject(double s[8], const StepParams& P)
 // scattering) only reserves its five draws and keeps -T in freq (pend set);
 // the frequency is drawn when a transfer update first needs it, or after the
 // launch (resolve_deferred), where every photon draws it at once instead of
-// one or two lanes at a time.  On arrival the redshift factor eobs / eem is
-// returned in fr for the deferred photon.  Results are bit for bit those of
-// the immediate emission.
+// one or two lanes at a time.  The arrival of a deferred photon only sets
+// its phase here: its redshift factor (and the zeroing of an invalid one)
+// follows after the launch from the stored state (deferred_arrival_factor).
+// Results are bit for bit those of the immediate emission.  (kDefer is used
+// only with NoHook: the differentiable form does not defer.)
 //
 // The Kerr-Schild radius of the new position is computed once here, for the
 // fluid sample and the observer energy, and returned in r_out: no event
@@ -1352,7 +1373,7 @@ constexpr uint32_t kSampleTried = 1u, kSampleHit = 2u;
 
 template <int kSep, int kMixed = 0, class Hook = NoHook, int kDefer = 0>
 GRT_HD uint32_t post_step(Photon& ph, uint8_t phase0, double taken, const GridView& G,
-                      const StepParams& P, PhotonRng& rng, double& au, double& av, double& fr,
+                      const StepParams& P, PhotonRng& rng, double& au, double& av,
                       double& r_out, Hook& hook, bool active)
 {
     typedef typename std::conditional<kMixed != 0, float, double>::type T;
@@ -1427,17 +1448,15 @@ GRT_HD uint32_t post_step(Photon& ph, uint8_t phase0, double taken, const GridVi
     if (active && image && phase0 == kForward) {
         const int crossing = camera_crossing(P, ph.s, au, av);
         if (crossing > 0) {
-            const double eobs = observer_energy(P, ph.s, radius);
-            const double eem = ph.eem;
-            if (eobs > 0.0 && eem > 0.0 && is_finite(eobs) && is_finite(eem)) {
-                if (kDefer && ph.pend)
-                    fr = eobs / eem;
-                else
+            if (!(kDefer && ph.pend)) {
+                const double eobs = observer_energy(P.a, P.m, ph.s, radius);
+                const double eem = ph.eem;
+                if (eobs > 0.0 && eem > 0.0 && is_finite(eobs) && is_finite(eem)) {
                     ph.freq *= eobs / eem;
-            } else {
-                ph.stokes[0] = ph.stokes[1] = ph.stokes[2] = ph.stokes[3] = 0.0;
-                hook.reset();
-                if (kDefer && ph.pend) fr = 1.0;
+                } else {
+                    ph.stokes[0] = ph.stokes[1] = ph.stokes[2] = ph.stokes[3] = 0.0;
+                    hook.reset();
+                }
             }
             ph.phase = kArrived;
             ph.term = 1;
@@ -1583,7 +1602,7 @@ GRT_HD void lane_reset(Lane& L, const StepParams& P)
 template <int kIntegrator, int kSep, class Rhs = RhsExact, int kMixed = 0, class Hook = NoHook,
           int kDefer = 0>
 GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, PhotonRng& rng,
-                         LaneCounters& n, double& au, double& av, double& fr, Hook& hook)
+                         LaneCounters& n, double& au, double& av, Hook& hook)
 {
     Photon& ph = L.ph;
     if (!L.in_call) {
@@ -1595,7 +1614,7 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
             rk4_step<Rhs>(ph.s, taken, P);
             ++n.accepted;
             n.count_sample(post_step<kSep, kMixed, Hook, kDefer>(ph, L.phase0, taken, G, P, rng, au,
-                                                                  av, fr, L.r_open, hook, true));
+                                                                  av, L.r_open, hook, true));
             return true;
         }
         L.attempt = 0;
@@ -1608,7 +1627,7 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
     const bool accepted = rk45_attempt<Rhs, kMixed>(ph.s, &ph.dl, k1, P);
     ++L.attempt;
     const uint32_t code = post_step<kSep, kMixed, Hook, kDefer>(ph, L.phase0, taken, G, P, rng, au,
-                                                                av, fr, L.r_open, hook, accepted);
+                                                                av, L.r_open, hook, accepted);
     if (accepted) {
         ++n.accepted;
         L.in_call = false;
@@ -1627,11 +1646,11 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
 
 template <int kIntegrator, int kSep, class Rhs = RhsExact, int kMixed = 0, int kDefer = 0>
 GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, PhotonRng& rng,
-                         LaneCounters& n, double& au, double& av, double& fr)
+                         LaneCounters& n, double& au, double& av)
 {
     NoHook h;
     return lane_iterate<kIntegrator, kSep, Rhs, kMixed, NoHook, kDefer>(L, G, P, rng, n, au, av,
-                                                                        fr, h);
+                                                                        h);
 }
 
 }  // namespace grt

@@ -26,7 +26,6 @@ struct PhotonPtrs {
     double* vdisp;
     uint32_t* rng_draw;
     uint8_t* fpend;     // library-owned: thermal frequency still to draw
-    double* fratio;     // library-owned: its arrival redshift factor
 };
 
 enum StatSlot {
@@ -64,7 +63,6 @@ struct TraceScratch {
     size_t list_cap;
     uint32_t* rng;        // per-photon RNG draw counters
     uint8_t* fpend;       // per-photon deferred-frequency flags
-    double* fratio;       // per-photon arrival redshift factors (deferred photons)
     uint32_t* counters;   // kReadbackBytes: driver counters, then the statistics
     TraceConsts* consts;  // launch constants
     bool fill_kernel;     // clear with launch_zero rather than hipMemsetAsync
@@ -181,7 +179,7 @@ hipError_t launch_build_live(const uint8_t* term, uint64_t n, uint32_t* list, ui
                              hipStream_t stream);
 // Draws the deferred thermal frequencies after a launch (grt_kernels.hip).
 hipError_t launch_resolve_deferred(const PhotonPtrs& ph, uint64_t n, uint64_t seed,
-                                   uint32_t stream_id, hipStream_t stream);
+                                   uint32_t stream_id, double a, double m, hipStream_t stream);
 hipError_t launch_image_products(const PhotonPtrs& ph, uint64_t n, int64_t nx, int64_t ny,
                                  int64_t bins, double smin, double smax, double lmin, double lmax,
                                  double width, double height, double* image, double* spec,

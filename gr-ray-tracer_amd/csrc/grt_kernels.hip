@@ -75,11 +75,13 @@ constexpr int kLdsBins = 4096;
 
 // build_observation_products, observation.hpp:31-84 (sums by atomics).
 // Draws the thermal frequencies whose draw a tracing launch deferred: the
-// five draws before the photon's saved draw counter, at T = -freq, then the
-// arrival redshift factor for a photon that reached the camera.  A photon
-// still in flight continues with the drawn frequency (fpend cleared).
+// five draws before the photon's saved draw counter, at T = -freq, then, for
+// a photon that reached the camera, the arrival's redshift factor from its
+// stored position and p_t (deferred_arrival_factor; an invalid one zeroes
+// the Stokes vector).  A photon still in flight continues with the drawn
+// frequency (fpend cleared).
 __global__ __launch_bounds__(256) void resolve_deferred(PhotonPtrs ph, uint64_t n, uint64_t seed,
-                                                        uint32_t stream)
+                                                        uint32_t stream, double a, double m)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !ph.fpend[i]) return;
@@ -88,7 +90,17 @@ __global__ __launch_bounds__(256) void resolve_deferred(PhotonPtrs ph, uint64_t 
     p.phase = ph.phase[i];
     p.pend = 1;
     const PhotonRng rng{ph.id[i], seed, stream, ph.rng_draw[i]};
-    finish_deferred(p, rng, p.phase == kArrived ? ph.fratio[i] : 1.0);
+    double fr = 1.0;
+    if (p.phase == kArrived) {
+        p.s[1] = ph.x[1][i];
+        p.s[2] = ph.x[2][i];
+        p.s[3] = ph.x[3][i];
+        p.s[4] = ph.k[0][i];
+        p.eem = ph.eem[i];
+        if (!deferred_arrival_factor(a, m, p, fr))
+            for (int q = 0; q < 4; ++q) ph.st[q][i] = 0.0;
+    }
+    finish_deferred(p, rng, fr);
     ph.freq[i] = p.freq;
     ph.fpend[i] = 0;
 }
@@ -221,13 +233,13 @@ hipError_t launch_build_live(const uint8_t* term, uint64_t n, uint32_t* list, ui
 }
 
 hipError_t launch_resolve_deferred(const PhotonPtrs& ph, uint64_t n, uint64_t seed,
-                                   uint32_t stream_id, hipStream_t stream)
+                                   uint32_t stream_id, double a, double m, hipStream_t stream)
 {
     if (n == 0) return hipSuccess;
     const int block = 256;
     const uint64_t blocks = (n + block - 1) / block;
     hipLaunchKernelGGL(resolve_deferred, dim3((unsigned)blocks), dim3(block), 0, stream, ph, n,
-                       seed, stream_id);
+                       seed, stream_id, a, m);
     return hipGetLastError();
 }
 

@@ -4,7 +4,7 @@
 //
 // The scratch is one file-scope __device__ array of fixed size: work lists,
 // per-photon RNG and deferral state, driver counters and launch constants
-// for at most GRT_NOALLOC_MAX_PHOTONS photons (about 21 MiB).  Every GPU the
+// for at most GRT_NOALLOC_MAX_PHOTONS photons (about 13 MiB).  Every GPU the
 // library runs on holds its own copy until the process ends.  In host code
 // the array's name is only a stand-in for its device address, so the address
 // of each device's copy is looked up once (hipGetSymbolAddress, with that
@@ -29,8 +29,7 @@ constexpr size_t kMaxN = GRT_NOALLOC_MAX_PHOTONS;
 constexpr size_t kOffLists = 0;
 constexpr size_t kOffRng = align_up(kOffLists + 2 * kMaxN * sizeof(uint32_t));
 constexpr size_t kOffFpend = align_up(kOffRng + kMaxN * sizeof(uint32_t));
-constexpr size_t kOffFratio = align_up(kOffFpend + kMaxN * sizeof(uint8_t));
-constexpr size_t kOffCounters = align_up(kOffFratio + kMaxN * sizeof(double));
+constexpr size_t kOffCounters = align_up(kOffFpend + kMaxN * sizeof(uint8_t));
 constexpr size_t kOffConsts = align_up(kOffCounters + kReadbackBytes);
 constexpr size_t kScratchBytes = align_up(kOffConsts + sizeof(TraceConsts));
 
@@ -86,7 +85,6 @@ hipError_t noalloc_scratch(int device, TraceScratch* s)
     s->list_cap = kMaxN;
     s->rng = reinterpret_cast<uint32_t*>(base + kOffRng);
     s->fpend = base + kOffFpend;
-    s->fratio = reinterpret_cast<double*>(base + kOffFratio);
     s->counters = reinterpret_cast<uint32_t*>(base + kOffCounters);
     s->consts = reinterpret_cast<TraceConsts*>(base + kOffConsts);
     s->fill_kernel = true;

@@ -153,7 +153,7 @@ __device__ __forceinline__ bool stage_take(WaveStage* S, StageState& st, const T
 }
 
 __device__ __forceinline__ void store_photon(const PhotonPtrs& p, uint32_t idx, const Photon& ph,
-                                             const PhotonRng& rng, double au, double av, double fr)
+                                             const PhotonRng& rng, double au, double av)
 {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -169,7 +169,6 @@ __device__ __forceinline__ void store_photon(const PhotonPtrs& p, uint32_t idx, 
     if (ph.phase == kArrived) {
         gmem(p.udisp)[idx] = au;
         gmem(p.vdisp)[idx] = av;
-        if (ph.pend) gmem(p.fratio)[idx] = fr;
     }
     gmem(p.rng_draw)[idx] = rng.draw;
     gmem(p.fpend)[idx] = ph.pend;
@@ -313,7 +312,7 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a_in)
         if (kSep) G.tab = lds;
         bool need = false;
         if (have) {
-            double au = 0.0, av = 0.0, fr = 1.0;
+            double au = 0.0, av = 0.0;
             if (kTape && !L.in_call && ((uint32_t)thdr % kSegmentCalls) == 0) {
                 const uint32_t k = (uint32_t)(thdr >> 32) & 0x7FFFFFFFu;
                 if (k < (uint32_t)a.tape_slots)
@@ -324,10 +323,10 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a_in)
             bool call_done;
             if (kTape)
                 call_done = lane_iterate<kIntegrator, kSep, Rhs, kMixed, TapeFlag>(
-                    L, G, P, rng, n, au, av, fr, flag);
+                    L, G, P, rng, n, au, av, flag);
             else
                 call_done = lane_iterate<kIntegrator, kSep, Rhs, kMixed, 1>(L, G, P, rng, n, au,
-                                                                            av, fr);
+                                                                            av);
             if (kTape && flag.touched) thdr |= kTapeActive;
             if (call_done) {
                 ++n.calls;
@@ -337,7 +336,7 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a_in)
                     thdr = tape_end_call(thdr, false, L.ph.term != 0, a.tape_slots,
                                          a.tape_overflow);
                 if (L.ph.term || calls_left == 0) {
-                    store_photon(PH, idx, L.ph, rng, au, av, fr);
+                    store_photon(PH, idx, L.ph, rng, au, av);
                     if (kTape) a.tape_hdr[idx] = thdr;
                     if (L.ph.term) {
                         const uint32_t step = (uint32_t)(a.step0 + calls_done - 1);

@@ -162,21 +162,27 @@ int harness_trace(const grt_params* p, int nr, int nth, int nph, const double* r
         lane_reset(L, sp);
         PhotonRng rng{P->id[i], sp.seed, sp.stream << 1, 0u};
         int calls = 0;
-        double au = 0.0, av = 0.0, fr = 1.0;
+        double au = 0.0, av = 0.0;
         // with the deferred thermal emission of the tracing kernel
         while (calls < max_steps && !f.term) {
             au = av = 0.0;
             bool done;
             const bool sep = pr.view.separable || !pr.view.available;
             if (sp.integrator == GRT_RK4)
-                done = sep ? lane_iterate<GRT_RK4, 1, RhsExact, 0, 1>(L, pr.view, sp, rng, cnt, au, av, fr)
-                           : lane_iterate<GRT_RK4, 0, RhsExact, 0, 1>(L, pr.view, sp, rng, cnt, au, av, fr);
+                done = sep ? lane_iterate<GRT_RK4, 1, RhsExact, 0, 1>(L, pr.view, sp, rng, cnt, au, av)
+                           : lane_iterate<GRT_RK4, 0, RhsExact, 0, 1>(L, pr.view, sp, rng, cnt, au, av);
             else
-                done = sep ? lane_iterate<GRT_RK45, 1, RhsExact, 0, 1>(L, pr.view, sp, rng, cnt, au, av, fr)
-                           : lane_iterate<GRT_RK45, 0, RhsExact, 0, 1>(L, pr.view, sp, rng, cnt, au, av, fr);
+                done = sep ? lane_iterate<GRT_RK45, 1, RhsExact, 0, 1>(L, pr.view, sp, rng, cnt, au, av)
+                           : lane_iterate<GRT_RK45, 0, RhsExact, 0, 1>(L, pr.view, sp, rng, cnt, au, av);
             if (done) { ++calls; ++cnt.calls; }
         }
-        if (f.pend) finish_deferred(f, rng, fr);
+        if (f.pend) {
+            // as resolve_deferred completes it after a tracing launch
+            double fr = 1.0;
+            if (f.phase == kArrived && !deferred_arrival_factor(sp.a, sp.m, f, fr))
+                f.stokes[0] = f.stokes[1] = f.stokes[2] = f.stokes[3] = 0.0;
+            finish_deferred(f, rng, fr);
+        }
         P->x0[i] = f.s[0]; P->x1[i] = f.s[1]; P->x2[i] = f.s[2]; P->x3[i] = f.s[3];
         P->k0[i] = f.s[4]; P->k1[i] = f.s[5]; P->k2[i] = f.s[6]; P->k3[i] = f.s[7];
         P->I[i] = f.stokes[0]; P->Q[i] = f.stokes[1]; P->U[i] = f.stokes[2]; P->V[i] = f.stokes[3];
