@@ -127,6 +127,28 @@ def test_c1_image(oracle, gpu, c1, nx, norm):
         assert np.all(g["phase"] == 4) and not g["I"].any()
 
 
+def test_arrival_inside_the_ergoregion(oracle, gpu, c1):
+    """A camera inside the ergoregion (equatorial, r = 1.8 < 2 M): most
+    arrivals have g_00 > 0, an observer energy of 0 and a zeroed Stokes
+    vector; the kernel completes these deferred arrivals after the launch
+    (resolve_deferred), the oracle at the arrival, and both agree."""
+    kw = c1_params(theta=np.pi / 2, camera_distance=1.8, camera_width=2.0,
+                   camera_height=2.0, r_term_max=300.0)
+    su, sv = pixel_screen(16, width=2.0, height=2.0)
+    ctx = gpu_ctx(kw, c1)
+    po, G = oracle_parts(oracle, kw, c1)
+    ph0 = oracle.init_image(po, su, sv, 0.01)
+    g, o, st_g, st_o = trace_both(oracle, ctx, po, G, ph0, max_steps=5000)
+    check_states(g, o, frac=1.0)
+    for k in ("I", "Q", "U", "V", "theta_disp", "phi_disp"):
+        assert np.array_equal(g[k], o[k]), k
+    # the emission-frame energy comes from the factored tetrad: 1e-14
+    f = np.abs(g["frequency"] - o["frequency"]) / np.maximum(np.abs(o["frequency"]), 1e-300)
+    assert f.max() <= 1e-14
+    arrived = o["phase"] == 3
+    assert (arrived & (o["I"] == 0)).sum() > 100 and (arrived & (o["I"] > 0)).sum() > 20
+
+
 @pytest.mark.parametrize("generator", [1, 2])
 def test_image_generators(oracle, gpu, c1, generator):
     """Image mode with the power-law generator (the emission event reads only

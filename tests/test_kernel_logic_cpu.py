@@ -112,6 +112,29 @@ def test_image_mode_scattering_never_fires(oracle, c1):
         assert np.array_equal(out[0][k], out[1][k]), k
 
 
+def ergoregion_camera_params():
+    """A camera inside the ergoregion (equatorial, r = 1.8 < 2 M for a = 0.9):
+    at most arrival points g_00 = 2H - 1 > 0, so camera_observer_energy is 0
+    and the arrival zeroes the Stokes vector (integrate_cart_ks.hpp:122-130,
+    392-400); slightly off the plane some arrivals stay valid."""
+    return c1_params(theta=np.pi / 2, camera_distance=1.8, camera_width=2.0,
+                     camera_height=2.0, r_term_max=300.0)
+
+
+def test_arrival_inside_the_ergoregion(oracle, c1):
+    """The kernel completes a deferred photon's arrival after the trace (its
+    redshift factor, or the zeroing of an invalid one, from the stored
+    state); the oracle does both at the arrival.  Both agree bit for bit,
+    on arrivals of both kinds."""
+    kw = ergoregion_camera_params()
+    su, sv = pixel_screen(16, width=2.0, height=2.0)
+    ph0 = oracle.init_image(oracle.make_params(**split_units(kw)[0]), su, sv, 0.01)
+    A, B, st_o, st_h = run(oracle, c1, kw, ph0, max_steps=5000)
+    assert_same(A, B, st_o, st_h)
+    arrived = A["phase"] == 3
+    assert (arrived & (A["I"] == 0)).sum() > 100 and (arrived & (A["I"] > 0)).sum() > 20
+
+
 def disk_packets(oracle, fields, kw):
     okw, units = split_units(kw)
     G = oracle.GridHolder(fields, kw["a_bh"], 1.0, units)
