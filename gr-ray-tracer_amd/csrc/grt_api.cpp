@@ -591,12 +591,12 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
     P.scatter = P.scatter && fluid;
 
     grt_stats st{};
-    // draw counters and deferred-frequency flags; counters and statistics
-    GRT_HIP(c, clear(S.rng, n * sizeof(uint32_t)));
-    GRT_HIP(c, clear(S.fpend, n));
+    // Counters and statistics; build_live also zeroes the draw counters and
+    // deferred-frequency flags.
     GRT_HIP(c, clear(S.counters, kReadbackBytes));
     uint32_t* list[2] = {S.lists, S.lists + S.list_cap};
-    GRT_HIP(c, grt::launch_build_live(dev->terminate, n, list[0], S.counters + 2, stream));
+    GRT_HIP(c, grt::launch_build_live(dev->terminate, n, list[0], S.counters + 2, S.rng, S.fpend,
+                                      stream));
     // The first launch reads the live count on the device; until its
     // readback, n bounds it.
     uint32_t live = (uint32_t)n;
@@ -692,7 +692,8 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
             L.n_total = (uint32_t)n;
             L.budget = budget;
             L.step0 = (int)step0;
-            GRT_HIP(c, clear(S.counters, 2 * sizeof(uint32_t)));
+            // (the first launch finds them zero from the clear above)
+            if (launches > 0) GRT_HIP(c, clear(S.counters, 2 * sizeof(uint32_t)));
             const int need_blocks = (int)((live + block - 1) / block);
             const int blocks = std::max(1, std::min(persistent_blocks, need_blocks));
             L.grid_lanes = (uint32_t)blocks * (uint32_t)block;

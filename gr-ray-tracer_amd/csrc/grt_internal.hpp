@@ -175,8 +175,10 @@ constexpr uint32_t kSegmentCalls = 32;   // functor calls per geodesic checkpoin
 constexpr int kCkptDoubles = 20;   // s[8], stokes[4], freq, eem, dl, phase, draw, call0, id, pad
 constexpr int kRecEntries = 64;    // replay records per lane: <= 2 per call
 constexpr int kRecDoubles = 16;    // cell (or -1: reset), I before, step, frequency, fluid[12]
+// The live list and its count, and the call's draw counters and
+// deferred-frequency flags set to zero (grt_kernels.hip).
 hipError_t launch_build_live(const uint8_t* term, uint64_t n, uint32_t* list, uint32_t* count,
-                             hipStream_t stream);
+                             uint32_t* draw, uint8_t* fpend, hipStream_t stream);
 // Draws the deferred thermal frequencies after a launch (grt_kernels.hip).
 hipError_t launch_resolve_deferred(const PhotonPtrs& ph, uint64_t n, uint64_t seed,
                                    uint32_t stream_id, double a, double m, hipStream_t stream);

@@ -23,11 +23,15 @@ __global__ __launch_bounds__(256, GRT_TRACE_WAVES_PER_EU) void trace_fp64(TraceL
 // Live list of the photons not yet terminated.  Each thread tests
 // kLiveItems consecutive entries; the block scans its counts and takes its
 // slice of the list with one atomic, so entries stay ascending within a block.
+// The same pass starts every photon's draw counter and deferred-frequency
+// flag of the call at zero (one launch instead of two fills before it).
 constexpr int kLiveItems = 8;
 
 __global__ __launch_bounds__(256) void build_live(const uint8_t* __restrict__ term, uint64_t n,
                                                   uint32_t* __restrict__ list,
-                                                  uint32_t* __restrict__ count)
+                                                  uint32_t* __restrict__ count,
+                                                  uint32_t* __restrict__ draw,
+                                                  uint8_t* __restrict__ fpend)
 {
     __shared__ uint32_t wave_total[4];
     __shared__ uint32_t block_base;
@@ -35,7 +39,11 @@ __global__ __launch_bounds__(256) void build_live(const uint8_t* __restrict__ te
     uint32_t live = 0;
 #pragma unroll
     for (int q = 0; q < kLiveItems; ++q)
-        if (first + q < n && !term[first + q]) live |= 1u << q;
+        if (first + q < n) {
+            if (!term[first + q]) live |= 1u << q;
+            draw[first + q] = 0u;
+            fpend[first + q] = 0;
+        }
     const uint32_t c = (uint32_t)__popc(live);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t incl = c;
@@ -221,14 +229,14 @@ hipError_t launch_trace(const TraceLaunch& a, int blocks, int block, size_t lds_
 }
 
 hipError_t launch_build_live(const uint8_t* term, uint64_t n, uint32_t* list, uint32_t* count,
-                             hipStream_t stream)
+                             uint32_t* draw, uint8_t* fpend, hipStream_t stream)
 {
     if (n == 0) return hipSuccess;
     const int block = 256;
     const uint64_t per_block = (uint64_t)block * kLiveItems;
     const uint64_t blocks = (n + per_block - 1) / per_block;
     hipLaunchKernelGGL(build_live, dim3((unsigned)blocks), dim3(block), 0, stream, term, n, list,
-                       count);
+                       count, draw, fpend);
     return hipGetLastError();
 }
 

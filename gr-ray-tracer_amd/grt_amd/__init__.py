@@ -214,7 +214,8 @@ def photon_struct(ph, device_index=None):
     s = Photons()
     n = len(ph["x0"])
     on_gpu = hasattr(ph["x0"], "data_ptr")
-    for name, dt in PHOTON_FIELDS:
+    fields = _torch_fields() if on_gpu else [(name, dt, None) for name, dt in PHOTON_FIELDS]
+    for name, dt, accepted in fields:
         a = ph[name]
         if hasattr(a, "data_ptr") != on_gpu:
             raise GrtError("photon fields must be all numpy arrays or all torch tensors "
@@ -223,10 +224,10 @@ def photon_struct(ph, device_index=None):
             if not a.is_cuda:
                 raise GrtError(f"photon field {name} is a CPU tensor; the device entry "
                                "points need GPU tensors (or pass numpy arrays)")
-            if device_index is not None and a.device.index != device_index:
+            if device_index is not None and a.get_device() != device_index:
                 raise GrtError(f"photon field {name} is on cuda:{a.device.index}, the "
                                f"context on cuda:{device_index}")
-            if a.dtype not in _torch_dtypes(dt):
+            if a.dtype not in accepted:
                 raise GrtError(f"photon field {name} must be {np.dtype(dt).name}, not {a.dtype}")
             size = a.numel() if a.dim() == 1 else -1
         else:
@@ -237,6 +238,18 @@ def photon_struct(ph, device_index=None):
             raise GrtError(f"photon field {name} must be one-dimensional of length {n}")
         setattr(s, name, _ptr(a))
     return s
+
+
+_TORCH_FIELDS = None
+
+
+def _torch_fields():
+    """(name, numpy dtype, accepted torch dtypes) of every photon field, built
+    once: photon_struct runs on every trace call."""
+    global _TORCH_FIELDS
+    if _TORCH_FIELDS is None:
+        _TORCH_FIELDS = [(name, dt, _torch_dtypes(dt)) for name, dt in PHOTON_FIELDS]
+    return _TORCH_FIELDS
 
 
 def _device_vector(t, what, n=None):
