@@ -483,33 +483,40 @@ GRT_HD double quarter_root(double x)
     return best;
 }
 
+// The residual c^4 - x of quarter_root's candidate check, with its exact
+// products, signed.
+GRT_HD double quarter_residual(double c, double x)
+{
+    const double s = c * c;
+    const double e1 = fma(c, c, -s);
+    const double q = s * s;
+    const double e2 = fma(s, s, -q);
+    return (q - x) + (e2 + 2.0 * s * e1);
+}
+
 // quarter_root(1.0 / err), bit for bit.  For err in (2^-500, 2^500) the
 // reciprocal and both roots take the short correctly rounded sequences and
 // the neighbours of y are its bit pattern -+ 1 (y is positive and finite
-// there), which is what nextafter returns; the candidates and their
-// residuals are those of quarter_root, in the same order.
+// there), which is what nextafter returns.  Of quarter_root's three
+// candidates only two can win: the residual evaluation is exact to about
+// 2^-101 x while neighbouring candidates' fourth powers differ by at least
+// 2^-51 x, so the residual grows strictly away from y on the side where y^4
+// already exceeds x (y^4 > x: y + ulp loses to y; y^4 < x: y - ulp loses).
+// The remaining neighbour is compared as quarter_root's loop would: y - ulp
+// is examined before y and keeps a tie, y + ulp after y and needs a strictly
+// smaller residual.
 GRT_HD double quarter_root_rcp(double err)
 {
 #if defined(__HIP_DEVICE_COMPILE__) && !defined(GRT_RELAXED_MATH)
     if (err > 0x1p-500 && err < 0x1p500) {
         const double x = rcp_rn(err);
         const double y = sqrt_rn(sqrt_rn(x));
-        const long long yb = __double_as_longlong(y);
-        double best = y, bestd = 1.0 / 0.0;
-#pragma unroll
-        for (int d = -1; d <= 1; ++d) {
-            const double c = d == 0 ? y : __longlong_as_double(yb + d);
-            const double s = c * c;
-            const double e1 = fma(c, c, -s);
-            const double q = s * s;
-            const double e2 = fma(s, s, -q);
-            const double r = fabs((q - x) + (e2 + 2.0 * s * e1));
-            if (r < bestd) {
-                bestd = r;
-                best = c;
-            }
-        }
-        return best;
+        const double ry = quarter_residual(y, x);
+        const bool above = ry > 0.0;
+        const double c = __longlong_as_double(__double_as_longlong(y) + (above ? -1 : 1));
+        const double rc = fabs(quarter_residual(c, x));
+        const double r0 = fabs(ry);
+        return (above ? rc <= r0 : rc < r0) ? c : y;
     }
 #endif
     return quarter_root(1.0 / err);

@@ -18,7 +18,9 @@
  *                                  same closed forms
  *   RK4 / RK45 ................... exact linear system (test_rk_integrators.cpp),
  *                                  convergence order + tolerance monotonicity
- *                                  (test_convergence.cpp), hand accept/reject cases
+ *                                  (test_convergence.cpp), hand accept/reject cases,
+ *                                  the Fehlberg stability polynomial and error constant
+ *   controller root x^(1/4) ...... round-to-nearest of the exact root (mpmath, 200 bits)
  *   cell search .................. 3x2x4 nonuniform grid -> (2,1,3)
  *                                  (test_npy_loader.cpp:74-97) + brute force
  *   tetrad ....................... g(e_a,e_b) = eta_ab, e0 || u
@@ -26,6 +28,15 @@
  *                                  test_radiative_transfer.cpp
  *   scattering ................... statistical (mu, phi uniform; E_loc kept;
  *                                  null norm), Stokes factors, Thomson P
+ *   photon generators ............ thermal: KS against the Planck energy spectrum and
+ *                                  its first two moments; power law: KS against the
+ *                                  analytic CDF (slopes 2.5, 1, 1+5e-13, -0.5); custom
+ *   disk emission ................ test_photon_generation.cpp:11-98; packet-to-cell
+ *                                  assignment against a brute-force expansion; positions
+ *                                  through the Boyer-Lindquist radius; fluid-frame isotropy
+ *   spherical -> Cartesian u, b .. central differences of the position map; g(u,u) = -1
+ *   global loop stop rule ........ flat-space (M = 0) rays with closed-form termination
+ *                                  steps, ten (threshold, interval, max_steps) cases
  *   camera / image products ...... flat-space (M = 0) straight rays hit the
  *                                  analytic screen point; test_output_stride.cpp
  *   Philox4x32-10 ................ Random123 known-answer vectors
@@ -373,6 +384,9 @@ static double quarter_root(double x)
     }
     return best;
 }
+
+/* The controller's root, exported for its test. */
+double orc_quarter_root(double x) { return quarter_root(x); }
 
 
 /* Rounding model of the integrators.  0 (default): the fused model that

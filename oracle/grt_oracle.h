@@ -116,6 +116,7 @@ int orc_rk45_generic(int n, double *state, double *dt, const double *k1, orc_rhs
                      double atol, double rtol, double min_s, double max_s, double safety);
 void orc_rk4_step_ks(double state[8], double dt, double a, double m);
 int orc_rk45_step_ks(const orc_params *p, double state[8], double *dlambda, const double k1[8]);
+double orc_quarter_root(double x);
 
 /* ---- fluid sampling, radiative_transfer/{scattering,fluid_frame}.hpp ---- */
 int orc_locate_cell(const orc_grid *g, double radius, double polar, double azimuth,

@@ -122,6 +122,7 @@ def lib():
                                  + [C.c_double] * 5),
             "orc_rk4_step_ks": (None, [_D, C.c_double, C.c_double, C.c_double]),
             "orc_rk45_step_ks": (C.c_int, [P, _D, _D, _D]),
+            "orc_quarter_root": (C.c_double, [C.c_double]),
             "orc_locate_cell": (C.c_int, [G, C.c_double, C.c_double, C.c_double,
                                           C.POINTER(C.c_int32)]),
             "orc_spherical_to_cartesian_vector": (None, [C.c_double] * 4 + [_D, _D]),
@@ -336,6 +337,11 @@ def rhs(X, p, a, m):
     out = np.zeros(8)
     lib().orc_hamiltonian_rhs(dptr(X), dptr(p), a, m, dptr(out))
     return out
+
+
+def quarter_root(x):
+    """The step controller's x^(1/4) (the candidate check of the oracle)."""
+    return lib().orc_quarter_root(float(x))
 
 
 def set_unfused_rounding(unfused):

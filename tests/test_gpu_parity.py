@@ -496,3 +496,21 @@ def test_correctly_rounded_sqrt_and_reciprocal(gpu):
     assert L.grt_math_check(1 << 27, 12345, v.ctypes.data, len(v), bad) == 0
     print("mismatches sqrt/rcp random", bad[0], bad[1], "edge", bad[2], bad[3])
     assert list(bad) == [0, 0, 0, 0]
+
+
+def test_controller_root_fast_path_equals_the_oracle_form(gpu):
+    """The FP64 kernel's step-size root quarter_root_rcp(err) (short correctly
+    rounded sequences, err in (2^-500, 2^500)) equals quarter_root(1.0 / err),
+    the oracle's candidate check with the device's sqrt and nextafter, bit for
+    bit on 2^27 pseudo-random err over that range (the oracle's form is itself
+    pinned against mpmath on the CPU)."""
+    import ctypes as C
+    import os
+    here = os.path.join(os.path.dirname(__file__), "harness", "libgrt_math_check.so")
+    L = C.CDLL(here)
+    L.grt_quarter_root_check.restype = C.c_int
+    L.grt_quarter_root_check.argtypes = [C.c_uint64, C.c_uint64, C.c_void_p]
+    bad = C.c_ulonglong(0)
+    assert L.grt_quarter_root_check(1 << 27, 4242, C.byref(bad)) == 0
+    print("quarter root mismatches", bad.value)
+    assert bad.value == 0

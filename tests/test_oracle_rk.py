@@ -117,6 +117,34 @@ def test_accept_reject_rules(oracle):
     assert not acc and s[0] == 1.0 and abs(hn - 0.01) < 1e-15   # scale clamped to min
 
 
+def test_controller_quarter_root_is_correctly_rounded(oracle):
+    """The step controller's (1/err)^(1/4) (integrate_cart_ks.hpp:217, pow(x,
+    0.25)) is the round-to-nearest value of the exact fourth root: against
+    mpmath at 200 bits on arguments spread over binary exponents [-960, 1000],
+    perfect fourth powers and their neighbours.  (Below 2^-960 the residual's
+    exact products reach the subnormal range, and near the largest double a
+    neighbour's fourth power overflows; the controller clamps such scales to
+    its step-scale bounds long before.)  sqrt(sqrt(x)) alone, the plausible
+    shortcut, misses the rounding on a share of the same arguments, so a
+    candidate check that is dropped or inverted fails here."""
+    import math
+    import mpmath
+    mpmath.mp.prec = 200
+    rng = np.random.default_rng(20261204)
+    xs = list(np.ldexp(rng.uniform(1.0, 2.0, 20000), rng.integers(-960, 1001, 20000)))
+    for k in (1.0, 3.0, 0.75, 1.5, 7.0, 1e5 + 1):
+        q = float(k) ** 4
+        xs += [q, np.nextafter(q, 0.0), np.nextafter(q, np.inf)]
+    xs += [2.0 ** -960, 2.0 ** 1000]
+    naive_wrong = 0
+    for x in xs:
+        exact = float(mpmath.root(mpmath.mpf(float(x)), 4))
+        got = oracle.quarter_root(x)
+        assert got == exact, (x, got, exact)
+        naive_wrong += math.sqrt(math.sqrt(x)) != exact
+    assert naive_wrong > 100
+
+
 def test_reference_convergence(oracle):
     # test_convergence.cpp:62-96 (a = M = 1)
     ok, p = oracle.null_momentum(np.array([0.0, 6.0, 2.0, 1.0]), [-0.4, 0.8, 0.2], 1.0, 1.0)
