@@ -42,6 +42,30 @@ GRT_HD V opaque(V v)
     return v;
 }
 
+// Wavefront priority of the tracing loop.  The two wavefronts that share a
+// SIMD alternate between a step's Runge-Kutta stages (long runs of
+// independent FP64 arithmetic) and the rest of the iteration: the step
+// controller, the post-step, the bookkeeping and the refills (short
+// dependency chains, branches and memory waits).  The rest runs at the higher
+// priority, so its next instruction, once ready, issues ahead of the other
+// wavefront's stage arithmetic, which fills the remaining issue slots; less
+// time then passes with both wavefronts outside their stages, when the
+// vector ALU idles.  (C2 render: 3% faster than equal priorities; the
+// opposite assignment is 5% slower.)  Results do not depend on it.
+GRT_HD void priority_stages()
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_s_setprio(0);
+#endif
+}
+
+GRT_HD void priority_control()
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_s_setprio(2);
+#endif
+}
+
 enum : uint8_t { kDisk = 0, kBackward = 1, kForward = 2, kArrived = 3, kRejected = 4 };
 
 // Constants captured by the step functor (integrate_cart_ks.hpp:35-61), with
@@ -598,6 +622,7 @@ GRT_HD bool rk45_attempt(double s[8], double* h_io, const double k1[8], const St
         X5[i] = fma(B55, k[i], X5[i]);
     }
     Rhs::eval(t, P, k);                                   // k6
+    priority_control();   // the error norm and the controller's root
 
     double x5[8];
     double err = 0.0;
@@ -1618,7 +1643,9 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
         if (kIntegrator != 1) {
             // RK4, integrate_cart_ks.hpp:269-272
             const double taken = ph.dl;
+            priority_stages();
             rk4_step<Rhs>(ph.s, taken, P);
+            priority_control();
             ++n.accepted;
             n.count_sample(post_step<kSep, kMixed, Hook, kDefer>(ph, L.phase0, taken, G, P, rng, au,
                                                                   av, L.r_open, hook, true));
@@ -1629,6 +1656,7 @@ GRT_HD bool lane_iterate(Lane& L, const GridView& G, const StepParams& P, Photon
     }
     // One attempt of the retry loop, integrate_cart_ks.hpp:273-299.
     double k1[8];
+    priority_stages();
     Rhs::eval(ph.s, P, k1);
     const double taken = ph.dl;
     const bool accepted = rk45_attempt<Rhs, kMixed>(ph.s, &ph.dl, k1, P);
