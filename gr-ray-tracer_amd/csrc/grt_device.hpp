@@ -752,6 +752,24 @@ GRT_HD int count_ge_desc(const double* t, int n, double v)
     return lo;
 }
 
+// The count of bucket [l, h) of an ascending (Desc: descending) table t that
+// is <= v (Desc: >= v), decided on the first two entries: t[l] and t[l + 1]
+// are read whatever h - l is (every table is followed by more tables in
+// LDS), and the rest of the bucket matters only when it has more than two
+// entries and the second one already counts, which the few crowded buckets
+// near the polar axis's ends can need.  Equal to count_le / count_ge_desc
+// of the bucket.
+template <bool Desc>
+GRT_HD int bucket_count_two(const double* t, int l, int h, double t0, double t1, double v)
+{
+    const bool in0 = Desc ? t0 >= v : t0 <= v;
+    const bool in1 = Desc ? t1 >= v : t1 <= v;
+    int c = l + ((h > l && in0) ? 1 : 0) + ((h > l + 1 && in1) ? 1 : 0);
+    if (h - l > 2 && in1)
+        c += Desc ? count_ge_desc(t + l + 2, h - l - 2, v) : count_le(t + l + 2, h - l - 2, v);
+    return c;
+}
+
 // count_le / count_ge_desc narrowed by the bucket counts B of the table
 // (axis_bucket of the keyed query w); equal to the plain counts.
 GRT_HD int bucket_count_le(const double* t, const uint16_t* B, int shift, int base, int nb,
@@ -759,7 +777,7 @@ GRT_HD int bucket_count_le(const double* t, const uint16_t* B, int shift, int ba
 {
     const int b = axis_bucket(w, shift, base, nb);
     const int l = B[b], h = B[b + 1];
-    return l + count_le(t + l, h - l, v);
+    return bucket_count_two<false>(t, l, h, t[l], t[l + 1], v);
 }
 
 GRT_HD int bucket_count_ge_desc(const double* t, const uint16_t* B, int shift, int base, int nb,
@@ -767,7 +785,7 @@ GRT_HD int bucket_count_ge_desc(const double* t, const uint16_t* B, int shift, i
 {
     const int b = axis_bucket(w, shift, base, nb);
     const int l = B[b], h = B[b + 1];
-    return l + count_ge_desc(t + l, h - l, v);
+    return bucket_count_two<true>(t, l, h, t[l], t[l + 1], v);
 }
 
 // A monotone stand-in for atan2(Y, X) mapped to [0, 2 pi): the "diamond
@@ -823,63 +841,70 @@ GRT_HD bool locate_separable(const GridView& G, double radius, double ct, double
     const uint16_t* BR = reinterpret_cast<const uint16_t*>(F + 4 * nph);
     const uint16_t* BT = BR + G.key_nb[0] + 1;
     const uint16_t* BF = BT + G.key_nb[1] + 1;
-    // radial (scattering.hpp:47-65)
     if (!separable_radial_windows(G, radius)) return false;
-    int i;
-    {
-        const int lo = (G.key_nb[0] > 0 ? bucket_count_le(R, BR, G.key_shift[0], G.key_base[0],
-                                                          G.key_nb[0], radius, radius)
-                                         : count_le(R, nr, radius)) - 1;
-        if (lo < 0) i = 0;
-        else if (lo >= nr - 1) i = nr - 1;
-        else i = (radius > R[nr + lo]) ? lo + 1 : lo;
+    // The three axes are independent: each step below reads all three
+    // axes' entries before the next step needs any of them, so a lookup
+    // costs about four dependent LDS round trips instead of twelve.
+    const double pa = pseudo_angle(X, Y);
+    int lo_r, lo_t, lo_f;   // brackets: entries <= r, >= cos theta, <= pseudo-angle, minus one
+    if (G.key_nb[0] > 0) {
+        const int br = axis_bucket(radius, G.key_shift[0], G.key_base[0], G.key_nb[0]);
+        const int bt = axis_bucket(3.0 - ct, G.key_shift[1], G.key_base[1], G.key_nb[1]);
+        const int bf = axis_bucket(pa + 4.0, G.key_shift[2], G.key_base[2], G.key_nb[2]);
+        const int lr = BR[br], hr = BR[br + 1];
+        const int lt = BT[bt], ht = BT[bt + 1];
+        const int lf = BF[bf], hf = BF[bf + 1];
+        const double r0 = R[lr], r1 = R[lr + 1];
+        const double t0 = T[lt], t1 = T[lt + 1];
+        const double f0 = F[lf], f1 = F[lf + 1];
+        lo_r = bucket_count_two<false>(R, lr, hr, r0, r1, radius) - 1;
+        lo_t = bucket_count_two<true>(T, lt, ht, t0, t1, ct) - 1;
+        lo_f = bucket_count_two<false>(F, lf, hf, f0, f1, pa) - 1;
+    } else {
+        lo_r = count_le(R, nr, radius) - 1;
+        lo_t = count_ge_desc(T, nth, ct) - 1;
+        lo_f = count_le(F, nph, pa) - 1;
     }
-    // within_* tests (scattering.hpp:99-149), each as soon as its index is
-    // known: the cell is only used when all three hold
-    if (!(R[2 * nr + i] <= radius && radius <= R[3 * nr + i])) return false;
-    // polar (scattering.hpp:67-81)
-    int j;
-    {
-        const int lo = (G.key_nb[1] > 0 ? bucket_count_ge_desc(T, BT, G.key_shift[1],
-                                                               G.key_base[1], G.key_nb[1],
-                                                               3.0 - ct, ct)
-                                         : count_ge_desc(T, nth, ct)) - 1;
-        if (lo < 0) j = 0;
-        else if (lo >= nth - 1) j = nth - 1;
-        else j = (ct < T[nth + lo]) ? lo + 1 : lo;
-    }
-    if (!(T[2 * nth + j] <= ct && ct <= T[3 * nth + j])) return false;
-    // azimuth (scattering.hpp:83-97): candidates 0, lo, lo+1, nph-1
+    // nearest entries (scattering.hpp:47-97, first strict minimum): the
+    // midpoint between the bracketing entries decides (radial and polar);
+    // the azimuth takes the largest dot product among candidates 0, lo,
+    // lo + 1 and nph - 1.  Entries are read at clamped indices and used only
+    // where the reference's scan would look.
+    const int mr = lo_r < 0 ? 0 : (lo_r > nr - 1 ? nr - 1 : lo_r);
+    const int mt = lo_t < 0 ? 0 : (lo_t > nth - 1 ? nth - 1 : lo_t);
+    const int c1 = lo_f, c2 = lo_f + 1, c3 = nph - 1;
+    const int q1 = c1 < 0 ? 0 : (c1 > nph - 1 ? nph - 1 : c1);
+    const int q2 = c2 < 0 ? 0 : (c2 > nph - 1 ? nph - 1 : c2);
+    const double mid_r = R[nr + mr], mid_t = T[nth + mt];
+    const double d0 = X * F[nph] + Y * F[2 * nph];
+    const double d1 = X * F[nph + q1] + Y * F[2 * nph + q1];
+    const double d2 = X * F[nph + q2] + Y * F[2 * nph + q2];
+    const double d3 = X * F[nph + c3] + Y * F[2 * nph + c3];
+    const int i = lo_r < 0 ? 0 : (lo_r >= nr - 1 ? nr - 1 : (radius > mid_r ? lo_r + 1 : lo_r));
+    const int j = lo_t < 0 ? 0 : (lo_t >= nth - 1 ? nth - 1 : (ct < mid_t ? lo_t + 1 : lo_t));
     int k = 0;
-    double best = X * F[nph] + Y * F[2 * nph];
-    {
-        const double pa = pseudo_angle(X, Y);
-        const int lo = (G.key_nb[2] > 0 ? bucket_count_le(F, BF, G.key_shift[2], G.key_base[2],
-                                                          G.key_nb[2], pa + 4.0, pa)
-                                         : count_le(F, nph, pa)) - 1;
-        const int c1 = lo, c2 = lo + 1, c3 = nph - 1;
-        if (c1 > 0 && c1 < nph) {
-            const double d = X * F[nph + c1] + Y * F[2 * nph + c1];
-            if (d > best) { best = d; k = c1; }
-        }
-        if (c2 > 0 && c2 > c1 && c2 < nph) {
-            const double d = X * F[nph + c2] + Y * F[2 * nph + c2];
-            if (d > best) { best = d; k = c2; }
-        }
-        if (c3 > 0 && c3 > c2) {
-            const double d = X * F[nph + c3] + Y * F[2 * nph + c3];
-            if (d > best) { best = d; k = c3; }
-        }
-    }
+    double best = d0;
+    if (c1 > 0 && c1 < nph && d1 > best) { best = d1; k = c1; }
+    if (c2 > 0 && c2 > c1 && c2 < nph && d2 > best) { best = d2; k = c2; }
+    if (c3 > 0 && c3 > c2 && d3 > best) { best = d3; k = c3; }
     i_out = i;
     j_out = j;
     k_out = k;
+    // within_* tests (scattering.hpp:99-149): the cell is used only when all
+    // three hold
+    const bool in_r = R[2 * nr + i] <= radius && radius <= R[3 * nr + i];
+    const bool in_t = T[2 * nth + j] <= ct && ct <= T[3 * nth + j];
     const double norm = sqrt(X * X + Y * Y);
     const double cthr = F[3 * nph + k];
-    if (cthr >= -1.0) return best >= cthr * norm;
-    // single azimuth: |sin d| <= 1e-10 on the near side
-    const double cross = X * F[2 * nph + k] - Y * F[nph + k];
-    return best >= 0.0 && fabs(cross) <= 1.0e-10 * norm;
+    bool in_f;
+    if (cthr >= -1.0) {
+        in_f = best >= cthr * norm;
+    } else {
+        // single azimuth: |sin d| <= 1e-10 on the near side
+        const double cross = X * F[2 * nph + k] - Y * F[nph + k];
+        in_f = best >= 0.0 && fabs(cross) <= 1.0e-10 * norm;
+    }
+    return in_r && in_t && in_f;
 }
 
 GRT_HD double c3at(const double* a, const GridView& G, int i, int j, int k)
