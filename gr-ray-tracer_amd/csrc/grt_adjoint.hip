@@ -70,14 +70,16 @@ struct RecordHook {
         put(1, Ib);
         put(2, taken);
         put(3, nu);
+#if GRT_GRADIENT_READS_GEOMETRY
         put(4, fl.position_cm[0]);
         put(5, fl.position_cm[1]);
         put(6, fl.position_cm[2]);
         put(7, fl.coordinate_time_s);
-        put(8, fl.density_g_cm3);
-        put(9, fl.temperature_k);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) put(10 + q, fl.magnetic_gauss[q]);
+        for (int q = 0; q < 4; ++q) put(8 + q, fl.magnetic_gauss[q]);
+#else
+        (void)fl;
+#endif
         ++count;
     }
     __device__ void reset()
@@ -86,21 +88,34 @@ struct RecordHook {
         put(0, -1.0);
         ++count;
     }
+    // The fluid of event e as the forward pass sampled it: density,
+    // temperature and four-velocity are the cell's own (scaled) record, which
+    // the tape's validity keeps unchanged; position, time and the magnetic
+    // field are recorded only where the gradient companions read them, and
+    // are NaN otherwise.
     __device__ FluidCellCGS fluid(int e, const double* cells, uint32_t cell) const
     {
         FluidCellCGS fl;
+        const double* c = cells + (size_t)cell * kCellDoubles;
+        fl.density_g_cm3 = c[0];
+        fl.temperature_k = c[1];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) fl.coordinate_four_velocity_cm_s[q] = c[2 + q];
+#if GRT_GRADIENT_READS_GEOMETRY
         fl.position_cm[0] = get(e, 4);
         fl.position_cm[1] = get(e, 5);
         fl.position_cm[2] = get(e, 6);
         fl.coordinate_time_s = get(e, 7);
-        fl.density_g_cm3 = get(e, 8);
-        fl.temperature_k = get(e, 9);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) fl.magnetic_gauss[q] = get(e, 10 + q);
-        // the four-velocity is the cell's own (scaled) record
-        const double* c = cells + (size_t)cell * kCellDoubles;
+        for (int q = 0; q < 4; ++q) fl.magnetic_gauss[q] = get(e, 8 + q);
+#else
+        (void)e;
+        const double nan = __builtin_nan("");
+        fl.position_cm[0] = fl.position_cm[1] = fl.position_cm[2] = nan;
+        fl.coordinate_time_s = nan;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) fl.coordinate_four_velocity_cm_s[q] = c[2 + q];
+        for (int q = 0; q < 4; ++q) fl.magnetic_gauss[q] = nan;
+#endif
         return fl;
     }
 };
@@ -133,6 +148,11 @@ __global__ __launch_bounds__(256, GRT_TRACE_WAVES_PER_EU) void adjoint_fp64(Adjo
     uint32_t p = wave_claim(a.queue);
     while (p < a.n) {
         double lam = a.seed[p];
+        // A photon's consecutive transfer steps mostly stay in one cell: the
+        // sweep sums a run of events in the same cell and scatter-adds it
+        // once, when the cell changes or the photon ends.
+        uint32_t run_cell = 0xFFFFFFFFu;
+        double run_rho = 0.0, run_T = 0.0;
         const uint64_t hdr = a.tape_hdr[p];
         const uint32_t calls_total = (uint32_t)hdr;
         for (int k = (int)((hdr >> 32) & 0x7FFFFFFFu) - 1; k >= 0 && lam != 0.0; --k) {
@@ -174,12 +194,24 @@ __global__ __launch_bounds__(256, GRT_TRACE_WAVES_PER_EU) void adjoint_fp64(Adjo
                     ComputeEmissivityIncrementGradient(fl, nu, taken, P.kemit, &di[0], &di[1],
                                                        &di[2]);
                 const double aI = att * Ib;
-                unsafeAtomicAdd(&a.grad_rho[cell], lam * (di[0] - aI * dt[0]) * a.density_scale);
-                unsafeAtomicAdd(&a.grad_T[cell], lam * (di[1] - aI * dt[1]) * a.temperature_scale);
+                if (cell != run_cell) {
+                    if (run_cell != 0xFFFFFFFFu) {
+                        unsafeAtomicAdd(&a.grad_rho[run_cell], run_rho);
+                        unsafeAtomicAdd(&a.grad_T[run_cell], run_T);
+                    }
+                    run_cell = cell;
+                    run_rho = run_T = 0.0;
+                }
+                run_rho += lam * (di[0] - aI * dt[0]) * a.density_scale;
+                run_T += lam * (di[1] - aI * dt[1]) * a.temperature_scale;
                 g_abs += lam * (-aI * dt[2]);
                 g_emit += lam * di[2];
                 lam *= att;
             }
+        }
+        if (run_cell != 0xFFFFFFFFu) {
+            unsafeAtomicAdd(&a.grad_rho[run_cell], run_rho);
+            unsafeAtomicAdd(&a.grad_T[run_cell], run_T);
         }
         p = wave_claim(a.queue);
     }

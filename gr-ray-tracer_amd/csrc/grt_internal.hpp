@@ -174,7 +174,10 @@ hipError_t launch_set_fluid_scalars(double* cells, double* rho_code, const doubl
 constexpr uint32_t kSegmentCalls = 32;   // functor calls per geodesic checkpoint
 constexpr int kCkptDoubles = 20;   // s[8], stokes[4], freq, eem, dl, phase, draw, call0, id, pad
 constexpr int kRecEntries = 64;    // replay records per lane: <= 2 per call
-constexpr int kRecDoubles = 16;    // cell (or -1: reset), I before, step, frequency, fluid[12]
+// cell (or -1: reset), I before, step, frequency; then, where the gradient
+// companions read them (GRT_GRADIENT_READS_GEOMETRY), position, time and the
+// magnetic field.  Density and temperature come back from the cell record.
+constexpr int kRecDoubles = GRT_GRADIENT_READS_GEOMETRY ? 12 : 4;
 // The live list and its count, and the call's draw counters and
 // deferred-frequency flags set to zero (grt_kernels.hip).
 hipError_t launch_build_live(const uint8_t* term, uint64_t n, uint32_t* list, uint32_t* count,

@@ -19,6 +19,14 @@
  * test suite checks each companion against central differences of its
  * increment.
  *
+ * GRT_GRADIENT_READS_GEOMETRY declares whether the two companions read
+ * fluid.position_cm, fluid.coordinate_time_s or fluid.magnetic_gauss.  The
+ * adjoint keeps those fields of every transfer step for its reverse sweep
+ * only when it is 1; with 0 (the default companions read only the density,
+ * the temperature and the step) they reach the companions as NaN, so a
+ * companion that reads them without the declaration gives NaN gradients,
+ * not wrong ones.  The increments themselves always see every field.
+ *
  * `real` is double in the FP64 path.  The RNG argument of
  * GeneratePhotonsCustom offers `double drand()` returning a uniform [0, 1)
  * draw from the per-photon counter-based stream.
@@ -31,6 +39,10 @@
 namespace grt_ext {
 using real = double;
 }
+
+#ifndef GRT_GRADIENT_READS_GEOMETRY
+#define GRT_GRADIENT_READS_GEOMETRY 0
+#endif
 
 struct FluidCellCGS {
     double position_cm[3] = {0.0, 0.0, 0.0};
