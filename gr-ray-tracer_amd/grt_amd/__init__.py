@@ -19,6 +19,9 @@ import numpy as np
 
 _PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.environ.get("GRT_AMD_LIB") or os.path.join(_PKG_ROOT, "libgrt_amd.so")
+# The build for ...Gradient companions that read the sample's position, time or
+# magnetic field (GRT_GRADIENT_READS_GEOMETRY=1), built beside the default one.
+GEOMETRY_LIB_PATH = os.path.join(_PKG_ROOT, "libgrt_amd_geometry.so")
 
 GRT_RK4, GRT_RK45 = 0, 1
 MODE_IMAGE, MODE_DISK = 0, 1
@@ -107,15 +110,24 @@ EXPORTED = [
 _lib = None
 
 
-def load():
-    """Load libgrt_amd.so (raises GrtError if it has not been built)."""
+def load(path=None):
+    """Load libgrt_amd.so (raises GrtError if it has not been built).  path:
+    another build of the library, such as GEOMETRY_LIB_PATH, loaded beside
+    it (not cached; pass the handle to Context)."""
     global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
-        raise GrtError(f"{LIB_PATH} is missing: build it with "
+    if path is not None:
+        return _bind(path)
+    if _lib is None:
+        _lib = _bind(LIB_PATH)
+    return _lib
+
+
+def _bind(lib_path):
+    """A handle on the library at lib_path with every entry point's signature."""
+    if not os.path.exists(lib_path):
+        raise GrtError(f"{lib_path} is missing: build it with "
                        "python gr-ray-tracer_amd/grt_amd/build.py")
-    L = C.CDLL(LIB_PATH)
+    L = C.CDLL(lib_path)
     P, PH, ST, TU = C.POINTER(Params), C.POINTER(Photons), C.POINTER(Stats), C.POINTER(Tuning)
     vp, dp = C.c_void_p, C.POINTER(C.c_double)
     sig = {
@@ -158,9 +170,8 @@ def load():
         f.restype = res
         f.argtypes = args
     if L.grt_abi_version() != ABI_VERSION:
-        raise GrtError(f"{LIB_PATH} has ABI version {L.grt_abi_version()}, this binding "
+        raise GrtError(f"{lib_path} has ABI version {L.grt_abi_version()}, this binding "
                        f"version {ABI_VERSION}: rebuild the library")
-    _lib = L
     return L
 
 
@@ -271,8 +282,9 @@ def _stats_dict(st):
 class Context:
     """A tracing context on the current HIP device (grt_create)."""
 
-    def __init__(self, params):
-        self._L = load()
+    def __init__(self, params, lib=None):
+        """lib: a handle from load(path) to run another build of the library."""
+        self._L = lib if lib is not None else load()
         h = C.c_void_p()
         rc = self._L.grt_create(C.byref(params), C.byref(h))
         if rc != 0:

@@ -16,6 +16,11 @@ REPO = os.path.dirname(ROOT)
 CSRC = os.path.join(ROOT, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 LIB_PATH = os.path.join(ROOT, "libgrt_amd.so")
+# The same library for gradient companions that read the sample's position,
+# time or magnetic field (GRT_GRADIENT_READS_GEOMETRY=1: the adjoint records
+# those fields too); built beside the default one so its record path is tested.
+GEOMETRY_LIB_PATH = os.path.join(ROOT, "libgrt_amd_geometry.so")
+GEOMETRY_SOURCES = ("grt_adjoint.hip", "grt_api.cpp")   # the sources the flag changes
 
 SOURCES = ["grt_kernels.hip", "grt_mixed.hip", "grt_adjoint.hip", "grt_disk.hip",
            "grt_noalloc.hip", "grt_api.cpp", "grt_io.cpp"]
@@ -79,6 +84,30 @@ def build(verbose=False, resource_usage=False, extra_flags=None, out=None):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc link of libgrt_amd.so failed")
     os.replace(tmp, lib_path)
+    if out is None and not extra_flags:
+        # the geometry-recording variant: the sources the flag changes are
+        # compiled again, the rest shared
+        gobjs = []
+        for src, obj in zip(SOURCES, objs):
+            if src in GEOMETRY_SOURCES:
+                gobj = os.path.join(build_dir, src + ".geometry.o")
+                contract = "fast" if src in RELAXED_FLAGS else "off"
+                cmd = [cc] + compile_flags(False, contract) + RELAXED_FLAGS.get(src, []) + \
+                    ["-DGRT_GRADIENT_READS_GEOMETRY=1", "-c", os.path.join(CSRC, src), "-o", gobj]
+                r = subprocess.run(cmd, capture_output=True, text=True)
+                if r.returncode != 0:
+                    sys.stderr.write(r.stdout + r.stderr)
+                    raise RuntimeError(f"hipcc failed on {src} (geometry records)")
+                gobjs.append(gobj)
+            else:
+                gobjs.append(obj)
+        gtmp = GEOMETRY_LIB_PATH + ".tmp"
+        r = subprocess.run([cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", gtmp] + gobjs,
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("hipcc link of libgrt_amd_geometry.so failed")
+        os.replace(gtmp, GEOMETRY_LIB_PATH)
     return lib_path
 
 

@@ -32,6 +32,17 @@ def test_library_exports_every_declared_symbol():
     assert L.grt_abi_version() == 2
 
 
+def test_geometry_recording_build_exports_the_same_entry_points():
+    """The build for gradient companions that read the sample's position, time
+    or magnetic field (GRT_GRADIENT_READS_GEOMETRY=1) is the same C ABI."""
+    import grt_amd
+    L = grt_amd.load(grt_amd.GEOMETRY_LIB_PATH)
+    for n in declared_functions():
+        assert hasattr(L, n), n
+    assert L.grt_abi_version() == grt_amd.load().grt_abi_version()
+    assert b"gfx950" in open(grt_amd.GEOMETRY_LIB_PATH, "rb").read()
+
+
 def test_library_is_in_tree_and_built_for_gfx950():
     import grt_amd
     assert os.path.dirname(grt_amd.LIB_PATH) == os.path.join(REPO, "gr-ray-tracer_amd")

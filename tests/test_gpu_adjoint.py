@@ -66,6 +66,35 @@ def test_adjoint_matches_reference_gradient(oracle, gpu, integrator):
     assert rel_close(gc, gc_o, 1e-10, 0.0)
 
 
+def test_geometry_recording_build_gives_the_same_gradient(oracle, gpu):
+    """The library built with GRT_GRADIENT_READS_GEOMETRY=1 (its adjoint also
+    records each transfer step's position, time and magnetic field, for
+    gradient companions that read them) returns the default build's gradient
+    with the default companions, which read neither (summation order of the
+    atomics aside)."""
+    import torch
+    import grt_amd
+    fields = small_torus()
+    kw, po, ph0 = disk_setup(oracle, fields)
+    n = len(ph0["x0"])
+    w = torch.from_numpy(np.random.default_rng(7).uniform(0.5, 1.5, n)).to(gpu)
+    out = []
+    for lib in (None, grt_amd.load(grt_amd.GEOMETRY_LIB_PATH)):
+        ctx = grt_amd.Context(grt_amd.default_params(**kw), lib=lib)
+        ctx.set_grid(fields)
+        g = grt_amd.to_device(ph0)
+        ctx.trace_taped(g, 4000, slots_per_photon=256)
+        out.append([t.cpu().numpy() for t in ctx.adjoint(w)])
+        ctx.close()
+    (gr0, gt0, gc0), (gr1, gt1, gc1) = out
+    print("nonzero cells", np.count_nonzero(gr0), "coeff", gc0, gc1)
+    assert np.count_nonzero(gr0) > 10
+    assert np.array_equal(gr0 != 0, gr1 != 0)
+    assert rel_close(gr1, gr0, 1e-12, 1e-14 * np.abs(gr0).max())
+    assert rel_close(gt1, gt0, 1e-12, 1e-14 * np.abs(gt0).max())
+    assert rel_close(gc1, gc0, 1e-12, 0.0)
+
+
 def test_adjoint_matches_gpu_finite_differences(oracle, gpu):
     import torch
     import grt_amd
