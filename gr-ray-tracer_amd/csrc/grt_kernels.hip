@@ -36,14 +36,29 @@ __global__ __launch_bounds__(256) void build_live(const uint8_t* __restrict__ te
     __shared__ uint32_t wave_total[4];
     __shared__ uint32_t block_base;
     const uint64_t first = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * kLiveItems;
-    uint32_t live = 0;
+    // the block's slice of the counters and flags, cleared with consecutive
+    // lanes on consecutive entries
+    const uint64_t slice = (uint64_t)blockIdx.x * blockDim.x * kLiveItems;
 #pragma unroll
-    for (int q = 0; q < kLiveItems; ++q)
-        if (first + q < n) {
-            if (!term[first + q]) live |= 1u << q;
-            draw[first + q] = 0u;
-            fpend[first + q] = 0;
+    for (int q = 0; q < kLiveItems; ++q) {
+        const uint64_t i = slice + (uint64_t)q * blockDim.x + threadIdx.x;
+        if (i < n) {
+            draw[i] = 0u;
+            fpend[i] = 0;
         }
+    }
+    uint32_t live = 0;
+    if (first + kLiveItems <= n && (reinterpret_cast<uintptr_t>(term) & 7u) == 0) {
+        // the thread's eight flags in one load
+        const uint64_t w = *reinterpret_cast<const uint64_t*>(term + first);
+#pragma unroll
+        for (int q = 0; q < kLiveItems; ++q)
+            if (((w >> (8 * q)) & 0xFFu) == 0) live |= 1u << q;
+    } else {
+#pragma unroll
+        for (int q = 0; q < kLiveItems; ++q)
+            if (first + q < n && !term[first + q]) live |= 1u << q;
+    }
     const uint32_t c = (uint32_t)__popc(live);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t incl = c;
