@@ -575,7 +575,6 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
         S.rng = c->d_rng;
         S.fpend = c->d_fpend;
         S.counters = c->d_counters;
-        S.consts = c->d_consts;
         S.fill_kernel = false;
     }
     unsigned long long* d_stats = reinterpret_cast<unsigned long long*>(S.counters + kCounterWords);
@@ -605,10 +604,8 @@ int trace_impl(grt_ctx* c, const grt_photons* dev, uint64_t n, int32_t max_steps
     c->h_consts.ph.fpend = S.fpend;
     c->h_consts.P = P;
     c->h_consts.grid = make_grid_view(c);
-    GRT_HIP(c, hipMemcpyAsync(S.consts, &c->h_consts, sizeof c->h_consts,
-                              hipMemcpyHostToDevice, stream));
     grt::TraceLaunch L{};
-    L.consts = S.consts;
+    L.cst = c->h_consts;   // carried in the kernel arguments
     L.integrator = P.integrator;
     L.grid = make_grid_view(c);
     L.queue = S.counters + 0;

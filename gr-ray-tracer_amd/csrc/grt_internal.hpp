@@ -39,9 +39,11 @@ enum StatSlot {
     kStatCount
 };
 
-// Launch constants, in device memory.  The tracing kernels read them through
-// a constant-address-space pointer (scalar loads, no wait on the kernel's
-// own photon stores); the adjoint reads the copy the last trace left.
+// Launch constants.  A tracing launch carries them at the start of its
+// kernel arguments (TraceLaunch::cst), which the kernel reads through a
+// constant-address-space pointer to its argument segment (scalar loads, no
+// copy before the launch, no wait on the kernel's own photon stores); the
+// adjoint reads a device copy.
 struct TraceConsts {
     PhotonPtrs ph;
     StepParams P;
@@ -51,10 +53,12 @@ struct TraceConsts {
 // Counters of the tracing driver: [0] work queue, [1] photons still live
 // after a launch, [2] live count before the first launch, [3] tape
 // overflow, [4] adjoint queue; then kStatCount u64 statistics (the word
-// count keeps them 8-byte aligned).  Read back together in one copy.
+// count keeps them 8-byte aligned), the block padded to a multiple of 16
+// bytes (the runtime clears a block that is not with two fill kernels).
+// Read back together in one copy.
 constexpr size_t kCounterWords = 8;
 constexpr size_t kReadbackBytes =
-    kCounterWords * sizeof(uint32_t) + kStatCount * sizeof(unsigned long long);
+    (kCounterWords * sizeof(uint32_t) + kStatCount * sizeof(unsigned long long) + 15) & ~size_t(15);
 
 // The device scratch of one trace call: the context's own allocations, or
 // the fixed array of the no-allocation entry point (grt_noalloc.hip).
@@ -64,7 +68,6 @@ struct TraceScratch {
     uint32_t* rng;        // per-photon RNG draw counters
     uint8_t* fpend;       // per-photon deferred-frequency flags
     uint32_t* counters;   // kReadbackBytes: driver counters, then the statistics
-    TraceConsts* consts;  // launch constants
     bool fill_kernel;     // clear with launch_zero rather than hipMemsetAsync
 };
 
@@ -107,7 +110,8 @@ inline size_t trace_lds_bytes(int lds_doubles, int block)
 }
 
 struct TraceLaunch {
-    const TraceConsts* consts;
+    // first: the kernels find it at offset 0 of their argument segment
+    TraceConsts cst;
     GridView grid;
     int integrator;
     const uint32_t* list_in;  // nullptr: the identity list (slot i is photon i)

@@ -3,8 +3,9 @@
 // capture, tight serving loops, harnesses that forbid hipMalloc).
 //
 // The scratch is one file-scope __device__ array of fixed size: work lists,
-// per-photon RNG and deferral state, driver counters and launch constants
-// for at most GRT_NOALLOC_MAX_PHOTONS photons (about 13 MiB).  Every GPU the
+// per-photon RNG and deferral state and the driver counters for at most
+// GRT_NOALLOC_MAX_PHOTONS photons (about 13 MiB; the launch constants travel
+// in the kernel arguments).  Every GPU the
 // library runs on holds its own copy until the process ends.  In host code
 // the array's name is only a stand-in for its device address, so the address
 // of each device's copy is looked up once (hipGetSymbolAddress, with that
@@ -30,8 +31,7 @@ constexpr size_t kOffLists = 0;
 constexpr size_t kOffRng = align_up(kOffLists + 2 * kMaxN * sizeof(uint32_t));
 constexpr size_t kOffFpend = align_up(kOffRng + kMaxN * sizeof(uint32_t));
 constexpr size_t kOffCounters = align_up(kOffFpend + kMaxN * sizeof(uint8_t));
-constexpr size_t kOffConsts = align_up(kOffCounters + kReadbackBytes);
-constexpr size_t kScratchBytes = align_up(kOffConsts + sizeof(TraceConsts));
+constexpr size_t kScratchBytes = align_up(kOffCounters + kReadbackBytes);
 
 }  // namespace
 }  // namespace grt
@@ -86,7 +86,6 @@ hipError_t noalloc_scratch(int device, TraceScratch* s)
     s->rng = reinterpret_cast<uint32_t*>(base + kOffRng);
     s->fpend = base + kOffFpend;
     s->counters = reinterpret_cast<uint32_t*>(base + kOffCounters);
-    s->consts = reinterpret_cast<TraceConsts*>(base + kOffConsts);
     s->fill_kernel = true;
     return hipSuccess;
 }

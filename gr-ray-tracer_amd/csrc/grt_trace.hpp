@@ -60,7 +60,16 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
 // all 64 lanes of the wavefront active, so each photon array is read in one
 // coalesced pass, and the photons' Kerr-Schild radii (spin a) are computed
 // 64 at a time.
-__device__ __forceinline__ void stage_fill(WaveStage* S, const TraceLaunch& a,
+// The work list of a launch: list_in (nullptr: the identity list) of n_in
+// slots, claimed through the counter queue by grid_lanes lanes.
+struct WorkList {
+    const uint32_t* list_in;
+    uint32_t n_in;
+    uint32_t* queue;
+    uint32_t grid_lanes;
+};
+
+__device__ __forceinline__ void stage_fill(WaveStage* S, const WorkList& a,
                                            const PhotonPtrs& p, double spin, uint32_t base,
                                            uint32_t count)
 {
@@ -113,7 +122,7 @@ __device__ __forceinline__ uint32_t stage_read(const WaveStage* S, uint32_t pos,
 // would.  Must be called with all 64 lanes active (st stays wave-uniform).
 // Returns whether this lane received a photon (then ph, rng, idx and the
 // photon's radius r hold it).
-__device__ __forceinline__ bool stage_take(WaveStage* S, StageState& st, const TraceLaunch& a,
+__device__ __forceinline__ bool stage_take(WaveStage* S, StageState& st, const WorkList& a,
                                            const PhotonPtrs& p, double spin, bool need, Photon& ph,
                                            PhotonRng& rng, uint32_t& idx, double& r)
 {
@@ -256,10 +265,13 @@ template <int kIntegrator, int kSep, class Rhs, int kMixed, int kTape = 0>
 __device__ __forceinline__ void trace_body(const TraceLaunch& a_in)
 {
     extern __shared__ double lds[];
-    TraceLaunch a = a_in;
+    // a_in stays in the argument segment (its constants are read from there,
+    // see below); only the work list is resolved into registers
+    const TraceLaunch& a = a_in;
+    WorkList W{a_in.list_in, a_in.n_in, a_in.queue, a_in.grid_lanes};
     if (a_in.n_in_dev) {
-        a.n_in = *a_in.n_in_dev;
-        if (a.n_in == a_in.n_total) a.list_in = nullptr;
+        W.n_in = *a_in.n_in_dev;
+        if (W.n_in == a_in.n_total) W.list_in = nullptr;
     }
     GridView G = a.grid;
     if (kSep) {
@@ -275,8 +287,11 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a_in)
     // opaquely every iteration: scalar loads at their use sites, instead of
     // some 130 scalar registers held across the whole loop and spilled to
     // vector lanes.
+    // The constants are TraceLaunch::cst, at offset 0 of the kernel's
+    // argument segment (every tracing kernel takes one TraceLaunch).
     typedef __attribute__((address_space(4))) const TraceConsts ConstTC;
-    const ConstTC* c0 = (const ConstTC*)a.consts;
+    static_assert(offsetof(TraceLaunch, cst) == 0, "TraceLaunch::cst must come first");
+    const ConstTC* c0 = (const ConstTC*)__builtin_amdgcn_kernarg_segment_ptr();
     const StepParams& P = *(const StepParams*)&c0->P;
     const PhotonPtrs& PH = *(const PhotonPtrs*)&c0->ph;
 
@@ -293,7 +308,7 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a_in)
     rng.seed = P.seed;
     rng.stream = P.stream << 1;
     uint32_t idx = 0;
-    bool have = stage_take(S, st, a, PH, P.a, true, L.ph, rng, idx, L.r_open);
+    bool have = stage_take(S, st, W, PH, P.a, true, L.ph, rng, idx, L.r_open);
     uint64_t thdr = 0;
     if (kTape && have) thdr = a.tape_hdr[idx];
     lane_reset_calls(L);
@@ -350,7 +365,7 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a_in)
             }
         }
         if (__ballot(need) != 0) {
-            const bool got = stage_take(S, st, a, PH, P.a, need, L.ph, rng, idx, L.r_open);
+            const bool got = stage_take(S, st, W, PH, P.a, need, L.ph, rng, idx, L.r_open);
             if (need) {
                 have = got;
                 if (got) {

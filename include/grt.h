@@ -166,8 +166,7 @@ int grt_trace_device(grt_ctx *ctx, const grt_photons *dev, uint64_t n, int32_t m
  * reference: for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
  * Same arguments, semantics and results as grt_trace_device, but the work
- * lists, per-photon RNG state, counters and launch constants live in one
- * fixed device array of the library (about 13 MiB, held by every GPU the
+ * lists, per-photon RNG state and counters live in one fixed device array of the library (about 13 MiB, held by every GPU the
  * library runs on until the process ends), so at most
  * GRT_NOALLOC_MAX_PHOTONS photons per call (GRT_E_RANGE beyond).  The call
  * makes no hipMalloc, hipFree or other allocating call (the grid, which
