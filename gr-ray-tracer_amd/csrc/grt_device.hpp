@@ -97,6 +97,8 @@ struct StepParams {
 // ---------------------------------------------------------------------------
 
 GRT_HD double sqrt_ieee(double x);   // IEEE sqrt, defined with the exact sequences below
+GRT_HD double div_rn(double n, double d, double y);   // correctly rounded n / d, below
+GRT_HD double rcp_rn(double d);                       // correctly rounded 1 / d, below
 
 // kerr_schild_core.hpp:28-33
 GRT_HD double ks_radius(double x, double y, double z, double a)
@@ -108,12 +110,22 @@ GRT_HD double ks_radius(double x, double y, double z, double a)
     return sqrt_ieee(0.5 * (A + S));
 }
 
+// The radii for which ks_H and ks_l divide through correctly rounded
+// reciprocals (div_rn, with |a| < 2^150 as well): the divisors r, r^2 + a^2
+// and r^4 + a^2 z^2 (|z| <= r) then lie far inside the normal range, as in
+// ks_rhs.  Outside it the IEEE division is used.
+GRT_HD bool ks_div_range(double r)
+{
+    return r > 0x1p-150 && r < 0x1p150;
+}
+
 // kerr_schild_core.hpp:80-83 with l_0 = 1: the g_00 entry, 2 H l_0 l_0 - 1.
 GRT_HD double ks_H(double r, double z, double a, double m)
 {
     const double r2 = r * r;
     const double az = a * z;
     const double D = r2 * r * r + az * az;
+    if (ks_div_range(r) && fabs(a) < 0x1p150) return div_rn(m * (r2 * r), D, rcp_rn(D));
     return m * (r2 * r) / D;
 }
 
@@ -122,6 +134,13 @@ GRT_HD void ks_l(double x, double y, double z, double r, double a, double l[4])
 {
     const double B = r * r + a * a;
     l[0] = 1.0;
+    if (ks_div_range(r) && fabs(a) < 0x1p150) {
+        const double iB = rcp_rn(B);
+        l[1] = div_rn(r * x + a * y, B, iB);
+        l[2] = div_rn(r * y - a * x, B, iB);
+        l[3] = div_rn(z, r, rcp_rn(r));
+        return;
+    }
     l[1] = (r * x + a * y) / B;
     l[2] = (r * y - a * x) / B;
     l[3] = z / r;
@@ -794,15 +813,21 @@ GRT_HD int bucket_count_ge_desc(const double* t, const uint16_t* B, int shift, i
 }
 
 // A monotone stand-in for atan2(Y, X) mapped to [0, 2 pi): the "diamond
-// angle" in [0, 4), one division, no transcendental.
+// angle" in [0, 4), one division, no transcendental.  Quadrant by quadrant
+// (Y >= 0, X >= 0: Y / (X + Y), or 0 at the origin; Y >= 0, X < 0:
+// 1 + (-X) / (Y - X); both negative: 2 + (-Y) / (-X - Y); else
+// 3 + X / (X - Y)), the same operations as the host's table values; the
+// quadrant only selects the operands, so a wavefront whose lanes lie in
+// several quadrants divides once.
 GRT_HD double pseudo_angle(double X, double Y)
 {
-    if (Y >= 0.0) {
-        if (X >= 0.0) return (X + Y > 0.0) ? Y / (X + Y) : 0.0;
-        return 1.0 + (-X) / (Y - X);
-    }
-    if (X < 0.0) return 2.0 + (-Y) / (-X - Y);
-    return 3.0 + X / (X - Y);
+    const bool yp = Y >= 0.0, xp = X >= 0.0;
+    const double num = yp ? (xp ? Y : -X) : (xp ? X : -Y);
+    const double den = yp ? (xp ? X + Y : Y - X) : (xp ? X - Y : -X - Y);
+    const double base = yp ? (xp ? 0.0 : 1.0) : (xp ? 3.0 : 2.0);
+    const double q = num / den;
+    if (yp && xp) return den > 0.0 ? q : 0.0;
+    return base + q;
 }
 
 // Separable fast path of locate_structured_grid_cell (scattering.hpp:151-180);
