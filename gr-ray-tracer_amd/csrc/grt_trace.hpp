@@ -114,12 +114,21 @@ __device__ __forceinline__ uint32_t stage_read(const WaveStage* S, uint32_t pos,
     return S->idx[pos];
 }
 
+// Claims near the end of a work list are rounded up to a multiple of this
+// many slots (the rest stay in the stage for the wavefront's next takers).
+// Claiming exactly what the lanes need meant about 49,000 claims of a few
+// photons each in the last quarter of a 1024^2 image render, each an atomic
+// on the one work counter and a load pass; multiples of 4 or 8 ran the render
+// 3.3% faster, 16 2.6%, 2 1.9% (identical results).
+constexpr uint32_t kStageClaimRound = 8;
+
 // Hands the next work-list slots, in order, to the lanes with `need` set.
 // When the stage runs out it claims the next slots with one atomic on the
 // work counter and loads them: 64 at a time while more than two photons per
-// lane of the grid remain, then only as many as the wavefront needs, so the
-// end of the list is spread over the wavefronts as evenly as single claims
-// would.  Must be called with all 64 lanes active (st stays wave-uniform).
+// lane of the grid remain, then as many as the wavefront needs rounded up to
+// kStageClaimRound, so the end of the list is spread over the wavefronts
+// nearly as evenly as single claims would.  Must be called with all 64 lanes
+// active (st stays wave-uniform).
 // Returns whether this lane received a photon (then ph, rng, idx and the
 // photon's radius r hold it).
 __device__ __forceinline__ bool stage_take(WaveStage* S, StageState& st, const WorkList& a,
@@ -140,8 +149,9 @@ __device__ __forceinline__ bool stage_take(WaveStage* S, StageState& st, const W
         // blockDim would be reloaded from the dispatch packet here (a global
         // load and a wait inside the loop)
         const uint32_t lanes = a.grid_lanes;
+        const uint32_t short_claim = (total - avail + kStageClaimRound - 1) & ~(kStageClaimRound - 1);
         const uint32_t want =
-            ((uint64_t)st.end + 2ull * lanes < (uint64_t)a.n_in) ? 64u : total - avail;
+            ((uint64_t)st.end + 2ull * lanes < (uint64_t)a.n_in) ? 64u : short_claim;
         uint32_t c = 0;
         if ((threadIdx.x & 63) == 0) c = atomicAdd(a.queue, want);
         const uint32_t nbase = __builtin_amdgcn_readfirstlane(c);
