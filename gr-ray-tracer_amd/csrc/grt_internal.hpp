@@ -100,6 +100,8 @@ struct StageState {
     uint32_t next;      // next slot to hand out, base <= next <= end
     uint32_t end;       // one past the last staged slot, end <= base + 64
     uint32_t drained;   // the work counter has passed the end of the list
+    uint32_t iters;     // loop iterations of the wavefront so far
+    uint32_t taken;     // photons its lanes took after the first one each
 };
 
 // Dynamic LDS of a tracing launch: search tables, then one stage per wavefront.

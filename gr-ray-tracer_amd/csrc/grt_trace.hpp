@@ -115,20 +115,26 @@ __device__ __forceinline__ uint32_t stage_read(const WaveStage* S, uint32_t pos,
 }
 
 // Claims near the end of a work list are rounded up to a multiple of this
-// many slots (the rest stay in the stage for the wavefront's next takers).
-// Claiming exactly what the lanes need meant about 49,000 claims of a few
-// photons each in the last quarter of a 1024^2 image render, each an atomic
-// on the one work counter and a load pass; multiples of 4 or 8 ran the render
-// 3.3% faster, 16 2.6%, 2 1.9% (identical results).
+// many slots (the rest stay in the stage for the wavefront's next takers)
+// while the wavefront's lanes take photons quickly: at least one per two
+// loop iterations so far, so a photon left in the stage waits a few
+// iterations at most.  Claiming exactly what the lanes need meant about
+// 49,000 claims of a few photons each in the last quarter of a 1024^2 image
+// render (paths of about 40 attempts), each an atomic on the one work counter
+// and a load pass: rounding to 8 ran it 2.6-3.3% faster.  Where paths are
+// long (RK4 disk photons, up to a launch's 256 calls) the photons a
+// wavefront holds back wait for its own lanes while others idle, and
+// rounding there ran the launch 18% slower; the rate test keeps those claims
+// exact.  Results are the same either way.
 constexpr uint32_t kStageClaimRound = 8;
 
 // Hands the next work-list slots, in order, to the lanes with `need` set.
 // When the stage runs out it claims the next slots with one atomic on the
 // work counter and loads them: 64 at a time while more than two photons per
-// lane of the grid remain, then as many as the wavefront needs rounded up to
-// kStageClaimRound, so the end of the list is spread over the wavefronts
-// nearly as evenly as single claims would.  Must be called with all 64 lanes
-// active (st stays wave-uniform).
+// lane of the grid remain, then as many as the wavefront needs (rounded up to
+// kStageClaimRound, see there), so the end of the list is spread over the
+// wavefronts nearly as evenly as single claims would.  Must be called with
+// all 64 lanes active (st stays wave-uniform).
 // Returns whether this lane received a photon (then ph, rng, idx and the
 // photon's radius r hold it).
 __device__ __forceinline__ bool stage_take(WaveStage* S, StageState& st, const WorkList& a,
@@ -149,7 +155,9 @@ __device__ __forceinline__ bool stage_take(WaveStage* S, StageState& st, const W
         // blockDim would be reloaded from the dispatch packet here (a global
         // load and a wait inside the loop)
         const uint32_t lanes = a.grid_lanes;
-        const uint32_t short_claim = (total - avail + kStageClaimRound - 1) & ~(kStageClaimRound - 1);
+        const bool quick = 2ull * st.taken >= (uint64_t)st.iters;
+        const uint32_t short_claim =
+            quick ? (total - avail + kStageClaimRound - 1) & ~(kStageClaimRound - 1) : total - avail;
         const uint32_t want =
             ((uint64_t)st.end + 2ull * lanes < (uint64_t)a.n_in) ? 64u : short_claim;
         uint32_t c = 0;
@@ -308,7 +316,7 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a_in)
     // This wavefront's photon stage follows the tables.
     WaveStage* S = reinterpret_cast<WaveStage*>(lds + ((a.lds_doubles + 1) & ~1)) +
                    (threadIdx.x >> 6);
-    StageState st{0u, 0u, 0u, 0u};
+    StageState st{0u, 0u, 0u, 0u, 0u, 0u};
 
     LaneCounters n{0u, 0u, 0u, 0u, 0u, 0u};
     uint32_t last_term = 0;
@@ -374,7 +382,10 @@ __device__ __forceinline__ void trace_body(const TraceLaunch& a_in)
                 }
             }
         }
-        if (__ballot(need) != 0) {
+        ++st.iters;
+        const uint64_t need_mask = __ballot(need);
+        if (need_mask != 0) {
+            st.taken += (uint32_t)__popcll(need_mask);
             const bool got = stage_take(S, st, W, PH, P.a, need, L.ph, rng, idx, L.r_open);
             if (need) {
                 have = got;
