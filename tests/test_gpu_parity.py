@@ -200,6 +200,45 @@ def test_work_list_edges(oracle, gpu, c1):
     ctx.tuning = {"block": 256, "blocks_per_cu": 2, "sub_budget": 256}
 
 
+def test_stage_refills_over_many_photons_per_lane(oracle, gpu, c1):
+    """Four photons per lane: every lane takes several photons from its
+    wavefront's stage, which claims 64 slots while more than two photons per
+    lane remain and then short claims (rounded up to eight slots while the
+    wavefront's lanes take photons quickly, exact otherwise, as in the first
+    iterations of every launch), in one launch and in launches of five calls
+    that compact the survivors.  Some photons are terminated on entry, so the
+    first list is not the identity.  Every photon must end exactly as in the
+    oracle."""
+    import grt_amd
+    kw = c1_params()
+    su, sv = pixel_screen(256)
+    ctx = gpu_ctx(kw, c1)
+    po, G = oracle_parts(oracle, kw, c1)
+    ph0 = oracle.init_image(po, su, sv, 0.1)
+    ph0["terminate"][::97] = 1
+    o = {k: v.copy() for k, v in ph0.items()}
+    st_o = oracle.trace(po, G, o, 10000, 1.0, 0, 8)
+    import torch
+    lanes = torch.cuda.get_device_properties(0).multi_processor_count * 64
+    assert len(su) >= 4 * lanes > 0, (len(su), lanes)
+    for tune in ({"block": 64, "blocks_per_cu": 1, "sub_budget": 1048576},
+                 {"block": 64, "blocks_per_cu": 1, "sub_budget": 5}):
+        ctx.tuning = tune
+        g = grt_amd.to_device(ph0)
+        st_g = ctx.trace(g, 10000)
+        g = grt_amd.to_host(g)
+        check_states(g, o, frac=1.0)
+        for k in ("x0", "x1", "x2", "x3", "k0", "k1", "k2", "k3", "I", "Q", "dlambda"):
+            assert np.array_equal(g[k], o[k]), (tune, k)
+        assert np.allclose(g["frequency"], o["frequency"], rtol=1e-14, atol=0.0,
+                           equal_nan=True), tune
+        for k in ("functor_calls", "accepted_steps", "rejected_attempts", "global_steps",
+                  "terminated", "fluid_hits"):
+            assert st_g[k] == st_o[k], (tune, k)
+        assert st_g["launches"] > (1 if tune["sub_budget"] == 5 else 0)
+    ctx.tuning = {"block": 256, "blocks_per_cu": 2, "sub_budget": 256}
+
+
 def test_empty_batches(gpu, c1):
     """Zero photons: every entry point succeeds and does nothing (the
     reference's loops run over an empty photon view)."""
