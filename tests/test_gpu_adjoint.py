@@ -257,3 +257,46 @@ def test_autograd_two_forwards_one_backward(oracle, gpu):
     I2 = transfer_intensity(ctx, g0, rho * 1.01, tg, 1.0, 1.0, max_steps=4000)
     with pytest.raises(grt_amd.GrtError, match="another taped trace"):
         (I1.sum() + I2.sum()).backward()
+
+
+def test_taped_trace_with_several_photons_per_lane(oracle, gpu):
+    """About 43,000 disk photons on 64-thread workgroups, one per CU (the
+    launch never has more lanes than photons, so only more photons than
+    lanes make a lane take several through its wavefront's stage): the taped
+    kernel's photons equal the plain FP64 kernel's bit for bit, and the
+    gradient equals that of 256-thread workgroups, two per CU, where every
+    lane traces one photon (summation order of the atomics aside)."""
+    import torch
+    import grt_amd
+    fields = small_torus()
+    kw = c1_params(mode=1, superphoton_count_normalization=150.0, absorption_coefficient=0.05,
+                   emission_coefficient=0.3, integrator=1, seed=7)
+    ctx = gpu_setup(kw, fields)
+    n = ctx.disk_photon_count()
+    lanes = torch.cuda.get_device_properties(0).multi_processor_count * 64
+    assert n > 2 * lanes, (n, lanes)
+    ph0 = grt_amd.empty_photons(n, device=gpu)
+    ctx.init_disk(ph0, 0.1)
+    w = torch.from_numpy(np.random.default_rng(5).uniform(0.5, 1.5, n)).to(gpu)
+    small = {"block": 64, "blocks_per_cu": 1, "sub_budget": 1048576}
+    wide = {"block": 256, "blocks_per_cu": 2, "sub_budget": 1048576}
+    ctx.tuning = small
+    ref = {k: v.clone() for k, v in ph0.items()}
+    ctx.trace(ref, 4000)
+    grads = []
+    for tune in (small, wide):
+        ctx.tuning = tune
+        g = {k: v.clone() for k, v in ph0.items()}
+        ctx.trace_taped(g, 4000, slots_per_photon=256)
+        for k in ("x0", "x1", "x2", "x3", "k0", "k1", "k2", "k3", "phase", "I", "Q"):
+            assert torch.equal(g[k], ref[k]), (tune, k)
+        grads.append([t.cpu().numpy() for t in ctx.adjoint(w)])
+        del g
+    (gr0, gt0, gc0), (gr1, gt1, gc1) = grads
+    print("photons", n, "nonzero cells", np.count_nonzero(gr0), "coeff", gc0, gc1)
+    assert np.count_nonzero(gr0) > 100
+    assert np.array_equal(gr0 != 0, gr1 != 0)
+    assert rel_close(gr0, gr1, 1e-12, 1e-14 * np.abs(gr1).max())
+    assert rel_close(gt0, gt1, 1e-12, 1e-14 * np.abs(gt1).max())
+    assert rel_close(gc0, gc1, 1e-12, 0.0)
+    ctx.tuning = {"block": 256, "blocks_per_cu": 2, "sub_budget": 256}
