@@ -230,3 +230,52 @@ def test_mixed_scattering_statistics(oracle, gpu):
                         np.linspace(lo, hi, 11)))
     print("chi2 p-values against an independent sample", ps)
     assert min(ps) > 0.01
+
+
+@pytest.mark.parametrize("precision", [1, 2])
+def test_mixed_kernels_with_several_photons_per_lane(gpu, precision):
+    """The mixed kernel and its FP64 control on 65,536 rays: with 64-thread
+    workgroups, one per CU, every lane takes several photons through its
+    wavefront's stage (and the stage's claims shrink near the end of the
+    list); in one launch each photon must end exactly as when 256-thread
+    workgroups, two per CU, give every lane one photon (a photon's
+    arithmetic does not depend on where or when it runs).  In launches of
+    five calls the geodesics and Stokes I are again exact; the frequency of
+    a photon that crosses a launch boundary after its emission agrees to
+    1e-12 relative: its deferred thermal frequency is drawn after that
+    launch, so its arrival redshift is applied in the kernel, with this
+    form's relaxed reciprocal and root, instead of after the trace in FP64."""
+    import torch
+    import grt_amd
+    fields, kw, su, sv = precision_form(256)
+    ctx = grt_amd.Context(grt_amd.default_params(precision=precision, **kw))
+    ctx.set_grid(fields)
+    lanes = torch.cuda.get_device_properties(0).multi_processor_count * 64
+    assert len(su) >= 4 * lanes, (len(su), lanes)
+    ph0 = grt_amd.empty_photons(len(su), device=gpu)
+    ctx.init_image(ph0, 0.1, torch.from_numpy(su).to(gpu), torch.from_numpy(sv).to(gpu))
+    out = []
+    for tune in ({"block": 256, "blocks_per_cu": 2, "sub_budget": 1048576},
+                 {"block": 64, "blocks_per_cu": 1, "sub_budget": 1048576},
+                 {"block": 64, "blocks_per_cu": 1, "sub_budget": 5}):
+        ctx.tuning = tune
+        ph = {k: v.clone() for k, v in ph0.items()}
+        st = ctx.trace(ph, 20000)
+        out.append((grt_amd.to_host(ph), st))
+    (ref, st_ref), (one, st_one), (five, st_five) = out
+    assert st_one["launches"] == 1 and st_five["launches"] > 1
+    keys = ("x0", "x1", "x2", "x3", "k0", "k1", "k2", "k3", "phase", "I", "dlambda")
+    for k in keys + ("frequency",):
+        assert np.array_equal(one[k], ref[k], equal_nan=True), (precision, k)
+    for k in keys:
+        assert np.array_equal(five[k], ref[k], equal_nan=True), (precision, k)
+    f, fr = five["frequency"], ref["frequency"]
+    both = np.isfinite(fr) & (fr != 0)
+    assert np.array_equal(np.isfinite(f), np.isfinite(fr))
+    assert np.all(np.abs(f[both] - fr[both]) <= 1e-12 * np.abs(fr[both])), precision
+    print("frequencies differing in the five-call launches:", int((f[both] != fr[both]).sum()))
+    for st in (st_one, st_five):
+        for k in ("functor_calls", "accepted_steps", "rejected_attempts", "fluid_hits",
+                  "global_steps", "terminated"):
+            assert st[k] == st_ref[k], (precision, k)
+    ctx.close()
