@@ -123,7 +123,8 @@ struct TraceLaunch {
     // identity list instead of list_in if all n_total photons are live.
     const uint32_t* n_in_dev;
     uint32_t n_total;
-    uint32_t* queue;          // work counter over list_in, zeroed before launch
+    uint32_t* queue;          // work counter over the slots of list_in from grid_lanes on,
+                              // zeroed before launch
     uint32_t* list_out;       // photons still live after `budget` calls
     uint32_t* n_out;          // zeroed before launch
     unsigned long long* stats;  // kStatCount slots

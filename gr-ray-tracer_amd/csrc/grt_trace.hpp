@@ -129,8 +129,9 @@ __device__ __forceinline__ uint32_t stage_read(const WaveStage* S, uint32_t pos,
 constexpr uint32_t kStageClaimRound = 8;
 
 // Hands the next work-list slots, in order, to the lanes with `need` set.
-// When the stage runs out it claims the next slots with one atomic on the
-// work counter and loads them: 64 at a time while more than two photons per
+// When the stage runs out it claims the next slots (the first claim of a
+// launch statically, then with one atomic on the work counter) and loads
+// them: 64 at a time while more than two photons per
 // lane of the grid remain, then as many as the wavefront needs (rounded up to
 // kStageClaimRound, see there), so the end of the list is spread over the
 // wavefronts nearly as evenly as single claims would.  Must be called with
@@ -159,10 +160,21 @@ __device__ __forceinline__ bool stage_take(WaveStage* S, StageState& st, const W
         const uint32_t short_claim =
             quick ? (total - avail + kStageClaimRound - 1) & ~(kStageClaimRound - 1) : total - avail;
         const uint32_t want =
-            ((uint64_t)st.end + 2ull * lanes < (uint64_t)a.n_in) ? 64u : short_claim;
-        uint32_t c = 0;
-        if ((threadIdx.x & 63) == 0) c = atomicAdd(a.queue, want);
-        const uint32_t nbase = __builtin_amdgcn_readfirstlane(c);
+            (st.iters == 0 || (uint64_t)st.end + 2ull * lanes < (uint64_t)a.n_in) ? 64u
+                                                                                  : short_claim;
+        uint32_t nbase;
+        if (st.iters == 0) {
+            // The first claim of a launch is static: wavefront w of the grid
+            // takes slots [64 w, 64 w + 64), which cover [0, grid_lanes) once,
+            // and the work counter hands out the slots from grid_lanes on.
+            // (Every wavefront's first claim on the one counter at once held
+            // the start of a 1024^2 image render back by 1%.)
+            nbase = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64u;
+        } else {
+            uint32_t c = 0;
+            if ((threadIdx.x & 63) == 0) c = atomicAdd(a.queue, want);
+            nbase = __builtin_amdgcn_readfirstlane(c) + lanes;
+        }
         stage_fill(S, a, p, spin, nbase, want);
         const uint32_t slot = nbase + (rank - avail);
         if (need && rank >= avail && slot < a.n_in) {
