@@ -57,7 +57,7 @@ def algorithmic_flops(t):
 # rocprofv3 --pmc passes (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0, the factors a
 # plain 1 GiB device copy reads on the same counters); the profile summary
 # profiles/trace_fp64_rk45_fm_torus_256x128x128_1024px.md holds the commands.
-TRAFFIC_BYTES_DEFAULT = 330.6e6
+TRAFFIC_BYTES_DEFAULT = 338.0e6
 
 
 def workload(args):
