@@ -224,28 +224,38 @@ def photon_struct(ph, device_index=None):
     (a short, mistyped or host array would be read out of bounds by a kernel)."""
     s = Photons()
     n = len(ph["x0"])
-    on_gpu = hasattr(ph["x0"], "data_ptr")
-    fields = _torch_fields() if on_gpu else [(name, dt, None) for name, dt in PHOTON_FIELDS]
-    for name, dt, accepted in fields:
-        a = ph[name]
-        if hasattr(a, "data_ptr") != on_gpu:
-            raise GrtError("photon fields must be all numpy arrays or all torch tensors "
-                           f"(field {name} is not)")
-        if on_gpu:
-            if not a.is_cuda:
+    if hasattr(ph["x0"], "data_ptr"):
+        # torch tensors: one pass of cheap accessors per field (this runs on
+        # every trace call); get_device() is -1 for a CPU tensor
+        shape = (n,)
+        for name, dt, accepted in _torch_fields():
+            a = ph[name]
+            if not hasattr(a, "data_ptr"):
+                raise GrtError("photon fields must be all numpy arrays or all torch tensors "
+                               f"(field {name} is not)")
+            dev = a.get_device()
+            if dev < 0:
                 raise GrtError(f"photon field {name} is a CPU tensor; the device entry "
                                "points need GPU tensors (or pass numpy arrays)")
-            if device_index is not None and a.get_device() != device_index:
-                raise GrtError(f"photon field {name} is on cuda:{a.device.index}, the "
+            if device_index is not None and dev != device_index:
+                raise GrtError(f"photon field {name} is on cuda:{dev}, the "
                                f"context on cuda:{device_index}")
             if a.dtype not in accepted:
                 raise GrtError(f"photon field {name} must be {np.dtype(dt).name}, not {a.dtype}")
-            size = a.numel() if a.dim() == 1 else -1
-        else:
-            if not isinstance(a, np.ndarray) or a.dtype != dt:
-                raise GrtError(f"photon field {name} must be a numpy {np.dtype(dt).name} array")
-            size = a.shape[0] if a.ndim == 1 else -1
-        if size != n:
+            if a.shape != shape:
+                raise GrtError(f"photon field {name} must be one-dimensional of length {n}")
+            if not a.is_contiguous():
+                raise GrtError("photon tensors must be contiguous")
+            setattr(s, name, a.data_ptr())
+        return s
+    for name, dt in PHOTON_FIELDS:
+        a = ph[name]
+        if hasattr(a, "data_ptr"):
+            raise GrtError("photon fields must be all numpy arrays or all torch tensors "
+                           f"(field {name} is not)")
+        if not isinstance(a, np.ndarray) or a.dtype != dt:
+            raise GrtError(f"photon field {name} must be a numpy {np.dtype(dt).name} array")
+        if (a.shape[0] if a.ndim == 1 else -1) != n:
             raise GrtError(f"photon field {name} must be one-dimensional of length {n}")
         setattr(s, name, _ptr(a))
     return s
