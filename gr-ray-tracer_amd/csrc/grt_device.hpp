@@ -1185,10 +1185,12 @@ GRT_HD bool sample_fluid(const double s[8], double radius, bool enabled, const G
     fl.coordinate_time_s = s[0] * P.time_scale;
     fl.density_g_cm3 = c[0];
     fl.temperature_k = c[1];
-    if (!(fl.density_g_cm3 > 0.0) || !is_finite(fl.density_g_cm3) || !is_finite(fl.temperature_k))
-        return false;
+    // the four-velocity is read with the density and temperature, before
+    // their check, so the record costs one memory round trip, not two
 #pragma unroll
     for (int q = 0; q < 4; ++q) fl.coordinate_four_velocity_cm_s[q] = c[2 + q];
+    if (!(fl.density_g_cm3 > 0.0) || !is_finite(fl.density_g_cm3) || !is_finite(fl.temperature_k))
+        return false;
     g = ks_metric_at(s, radius, P.a, P.m);
     if (!fluid_tetrad<T>(g, fl.coordinate_four_velocity_cm_s, e, false)) return false;
     // The spatial legs of the tetrad serve the magnetic field and a
